@@ -1,0 +1,505 @@
+// api.hip - the C ABI of include/bhx.h: context, uploads, launches.
+#include "../../include/bhx.h"
+#include "host_setup.h"
+#include "params.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace bhx {
+hipError_t launch_trace(const TraceArgs& args, int grid, int block, hipStream_t stream);
+hipError_t trace_occupancy(int block, int* blocks_per_cu);
+hipError_t launch_shade(const ShadeArgs& args, int blocks, hipStream_t s);
+hipError_t launch_develop(const double* xyz, int64_t n, double exposure, int tm, double log_k,
+                          uint8_t* rgb, hipStream_t s);
+hipError_t launch_lum_hist(const double* xyz, int64_t n, int shift, unsigned long long prefix,
+                           unsigned int* hist, unsigned long long* n_pos, hipStream_t s);
+hipError_t launch_bb(const double* tab, const double* T, int64_t n, double* out, hipStream_t s);
+hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);
+hipError_t launch_zero_u32(unsigned int* p, int n, hipStream_t s);
+}  // namespace bhx
+
+using namespace bhx;
+
+// Small per-context device block: counters of one trace + one shade.
+struct Counters {
+    unsigned long long totals[4];       // steps, attempts, max norm bits, max Carter bits
+    unsigned long long classes[8];      // 6 classes + jet rays
+    unsigned int tile_counter;
+    unsigned int pad;
+};
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t n) {
+        if (n <= bytes) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) bytes = n;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+struct bhx_context {
+    int device = 0;
+    int n_cu = 0;
+    std::string err;
+    bhx_launch launch{256, 0, 64, 16};
+    // tables
+    double* d_bb = nullptr;
+    // catalogue (cell CSR + star rows)
+    bool custom_cat = false;
+    host::Catalogue custom;
+    bool cat_valid = false;
+    int cat_count = -1;
+    uint64_t cat_seed = 0;
+    bool cat_is_custom = false;
+    DevBuf cell_start, star_rows;
+    // tile order cache
+    int tiles_w = -1, tiles_h = -1;
+    DevBuf tile_order;
+    int n_tiles = 0;
+    // counters
+    Counters* d_cnt = nullptr;
+    DevBuf flux_partial, lum_hist, lum_npos;
+    // frame buffers of bhx_render
+    DevBuf r_cls, r_y, r_jet, r_xyz;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+namespace {
+
+int fail(bhx_context* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define HIPCHK(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(ctx, BHX_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int check_scene(bhx_context* ctx, const bhx_scene* s) {
+    if (!s) return fail(ctx, BHX_E_INVALID, "scene is NULL");
+    if (s->width <= 0 || s->height <= 0) return fail(ctx, BHX_E_INVALID, "width/height must be > 0");
+    const int S = std::max(1, s->sqrt_spp);
+    const double n = static_cast<double>(s->width) * s->height * S * S;
+    if (n > 2.0e9) return fail(ctx, BHX_E_TOO_LARGE, "more than 2e9 samples in one frame");
+    if (!(s->rtol > 0.0) || !(s->atol > 0.0)) return fail(ctx, BHX_E_INVALID, "rtol/atol must be > 0");
+    return BHX_OK;
+}
+
+// Centre-first order of 8 x 8 tiles: the photon ring (longest rays) sits near
+// the optical axis, so starting there shortens the queue's tail.
+int ensure_tiles(bhx_context* ctx, int W, int H) {
+    const int tw = (W + 7) / 8, th = (H + 7) / 8;
+    if (tw == ctx->tiles_w && th == ctx->tiles_h) return BHX_OK;
+    const int n = tw * th;
+    std::vector<int32_t> order(n);
+    std::vector<double> key(n);
+    for (int i = 0; i < n; ++i) {
+        order[i] = i;
+        const double cx = (i % tw) * 8 + 4 - 0.5 * W, cy = (i / tw) * 8 + 4 - 0.5 * H;
+        key[i] = cx * cx + cy * cy;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+    HIPCHK(ctx->tile_order.ensure(sizeof(int32_t) * n));
+    HIPCHK(hipMemcpy(ctx->tile_order.p, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    ctx->tiles_w = tw;
+    ctx->tiles_h = th;
+    ctx->n_tiles = n;
+    return BHX_OK;
+}
+
+int upload_catalogue(bhx_context* ctx, const host::Catalogue& cat) {
+    std::vector<uint32_t> start;
+    std::vector<double> rows;
+    host::bin_catalogue(cat, start, rows);
+    HIPCHK(ctx->cell_start.ensure(sizeof(uint32_t) * start.size()));
+    HIPCHK(hipMemcpy(ctx->cell_start.p, start.data(), sizeof(uint32_t) * start.size(), hipMemcpyHostToDevice));
+    HIPCHK(ctx->star_rows.ensure(std::max<size_t>(8, sizeof(double) * rows.size())));
+    if (!rows.empty())
+        HIPCHK(hipMemcpy(ctx->star_rows.p, rows.data(), sizeof(double) * rows.size(), hipMemcpyHostToDevice));
+    return BHX_OK;
+}
+
+int ensure_catalogue(bhx_context* ctx, const bhx_scene* s) {
+    if (!s->stars_enabled) return BHX_OK;
+    if (ctx->custom_cat) {
+        if (ctx->cat_valid && ctx->cat_is_custom) return BHX_OK;
+        int rc = upload_catalogue(ctx, ctx->custom);
+        if (rc) return rc;
+        ctx->cat_valid = true;
+        ctx->cat_is_custom = true;
+        return BHX_OK;
+    }
+    if (ctx->cat_valid && !ctx->cat_is_custom && ctx->cat_count == s->star_count &&
+        ctx->cat_seed == s->seed)
+        return BHX_OK;
+    host::Catalogue cat;
+    host::generate_catalogue(std::max(0, s->star_count), s->seed, cat);
+    int rc = upload_catalogue(ctx, cat);
+    if (rc) return rc;
+    ctx->cat_valid = true;
+    ctx->cat_is_custom = false;
+    ctx->cat_count = s->star_count;
+    ctx->cat_seed = s->seed;
+    return BHX_OK;
+}
+
+int trace_grid(bhx_context* ctx, int* grid, int* block) {
+    *block = ctx->launch.block > 0 ? ctx->launch.block : 256;
+    int per_cu = 0;
+    HIPCHK(trace_occupancy(*block, &per_cu));
+    if (per_cu <= 0) return fail(ctx, BHX_E_HIP, "tracer kernel cannot be resident");
+    if (ctx->launch.waves_per_simd > 0) {
+        const int want = std::max(1, ctx->launch.waves_per_simd * 4 * 64 / *block);
+        per_cu = std::min(per_cu, want);
+    }
+    *grid = per_cu * ctx->n_cu;
+    return BHX_OK;
+}
+
+double ms_between(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+    return ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bhx_abi_version(void) { return BHX_ABI_VERSION; }
+
+const char* bhx_last_error(const bhx_context* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int bhx_create(int device, bhx_context** out) {
+    if (!out) return BHX_E_INVALID;
+    *out = nullptr;
+    bhx_context* ctx = new bhx_context();
+    ctx->device = device;
+    auto bail = [&](int rc) {
+        bhx_destroy(ctx);
+        return rc;
+    };
+    if (hipSetDevice(device) != hipSuccess) return bail(BHX_E_HIP);
+    if (hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return bail(BHX_E_HIP);
+    const std::vector<double>& tab = host::bb_table();
+    if (hipMalloc(&ctx->d_bb, sizeof(double) * tab.size()) != hipSuccess) return bail(BHX_E_NOMEM);
+    if (hipMemcpy(ctx->d_bb, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(BHX_E_HIP);
+    if (hipMalloc(&ctx->d_cnt, sizeof(Counters)) != hipSuccess) return bail(BHX_E_NOMEM);
+    for (auto& e : ctx->ev)
+        if (hipEventCreate(&e) != hipSuccess) return bail(BHX_E_HIP);
+    *out = ctx;
+    return BHX_OK;
+}
+
+int bhx_destroy(bhx_context* ctx) {
+    if (!ctx) return BHX_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    if (ctx->d_bb) (void)hipFree(ctx->d_bb);
+    if (ctx->d_cnt) (void)hipFree(ctx->d_cnt);
+    for (DevBuf* b : {&ctx->cell_start, &ctx->star_rows, &ctx->tile_order, &ctx->flux_partial,
+                      &ctx->lum_hist, &ctx->lum_npos, &ctx->r_cls, &ctx->r_y, &ctx->r_jet, &ctx->r_xyz})
+        b->release();
+    for (auto& e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete ctx;
+    return BHX_OK;
+}
+
+int bhx_set_catalogue(bhx_context* ctx, int32_t n, const double* x, const double* y, const double* z,
+                      const double* T, const double* irr) {
+    if (!ctx) return BHX_E_INVALID;
+    if (n < 0) return fail(ctx, BHX_E_INVALID, "negative catalogue size");
+    ctx->cat_valid = false;
+    if (n == 0) {
+        ctx->custom_cat = false;
+        ctx->custom = host::Catalogue{};
+        return BHX_OK;
+    }
+    if (!x || !y || !z || !T || !irr) return fail(ctx, BHX_E_INVALID, "catalogue array is NULL");
+    ctx->custom_cat = true;
+    ctx->custom.x.assign(x, x + n);
+    ctx->custom.y.assign(y, y + n);
+    ctx->custom.z.assign(z, z + n);
+    ctx->custom.T.assign(T, T + n);
+    ctx->custom.irr.assign(irr, irr + n);
+    return BHX_OK;
+}
+
+int bhx_set_launch(bhx_context* ctx, const bhx_launch* l) {
+    if (!ctx || !l) return BHX_E_INVALID;
+    if (l->block < 0 || (l->block % 64) != 0 || l->block > 1024)
+        return fail(ctx, BHX_E_INVALID, "block must be a multiple of 64 up to 256");
+    if (l->block > 256) return fail(ctx, BHX_E_INVALID, "block must be at most 256");
+    if (l->refill < 0 || l->refill > 64) return fail(ctx, BHX_E_INVALID, "refill must be in [0, 64]");
+    if (l->waves_per_simd < 0 || l->waves_per_simd > 8)
+        return fail(ctx, BHX_E_INVALID, "waves_per_simd must be in [0, 8]");
+    if (l->block) ctx->launch.block = l->block;
+    if (l->waves_per_simd) ctx->launch.waves_per_simd = l->waves_per_simd;
+    if (l->chunk) ctx->launch.chunk = l->chunk;
+    if (l->refill) ctx->launch.refill = l->refill;
+    return BHX_OK;
+}
+
+int bhx_get_launch(const bhx_context* ctx, bhx_launch* l) {
+    if (!ctx || !l) return BHX_E_INVALID;
+    *l = ctx->launch;
+    return BHX_OK;
+}
+
+int bhx_trace_dev(bhx_context* ctx, const bhx_scene* s, const bhx_rays* rays, bhx_stats* stats,
+                  void* stream) {
+    if (!ctx) return BHX_E_INVALID;
+    int rc = check_scene(ctx, s);
+    if (rc) return rc;
+    if (!rays) return fail(ctx, BHX_E_INVALID, "rays is NULL");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    rc = ensure_tiles(ctx, s->width, s->height);
+    if (rc) return rc;
+    TraceArgs P = host::trace_args(*s);
+    P.tiles_x = ctx->tiles_w;
+    P.n_tiles = ctx->n_tiles;
+    P.refill = std::max(1, std::min(64, ctx->launch.refill));
+    P.tile_order = static_cast<const int32_t*>(ctx->tile_order.p);
+    P.tile_counter = &ctx->d_cnt->tile_counter;
+    P.n_rays = static_cast<int64_t>(s->width) * s->height * P.S * P.S;
+    P.cls = rays->cls;
+    P.y = rays->y;
+    P.jet = rays->jet;
+    P.steps = rays->steps;
+    P.attempts = rays->attempts;
+    P.norm_err = rays->norm_err;
+    P.carter = rays->carter;
+    P.totals = ctx->d_cnt->totals;
+    int grid = 0, block = 0;
+    rc = trace_grid(ctx, &grid, &block);
+    if (rc) return rc;
+    // counters are cleared by a fill kernel (never a memset node: the call may be captured)
+    HIPCHK(launch_zero_u64(reinterpret_cast<unsigned long long*>(ctx->d_cnt), sizeof(Counters) / 8, st));
+    HIPCHK(hipEventRecord(ctx->ev[0], st));
+    HIPCHK(launch_trace(P, grid, block, st));
+    HIPCHK(hipEventRecord(ctx->ev[1], st));
+    if (stats) {
+        HIPCHK(hipStreamSynchronize(st));
+        Counters c;
+        HIPCHK(hipMemcpy(&c, ctx->d_cnt, sizeof c, hipMemcpyDeviceToHost));
+        double mn, mc;
+        std::memcpy(&mn, &c.totals[2], 8);
+        std::memcpy(&mc, &c.totals[3], 8);
+        stats->rays += P.n_rays;
+        stats->steps += static_cast<int64_t>(c.totals[0]);
+        stats->attempts += static_cast<int64_t>(c.totals[1]);
+        stats->rhs_evals += static_cast<int64_t>(c.totals[1]) * 6 + P.n_rays;
+        stats->max_norm_error = std::max(stats->max_norm_error, mn);
+        stats->max_carter_drift = std::max(stats->max_carter_drift, mc);
+        stats->trace_ms = ms_between(ctx->ev[0], ctx->ev[1]);
+    }
+    return BHX_OK;
+}
+
+int bhx_shade_dev(bhx_context* ctx, const bhx_scene* s, const bhx_rays* rays, const bhx_shaded* out,
+                  bhx_stats* stats, void* stream) {
+    if (!ctx) return BHX_E_INVALID;
+    int rc = check_scene(ctx, s);
+    if (rc) return rc;
+    if (!rays || !rays->cls || !rays->y) return fail(ctx, BHX_E_INVALID, "rays.cls / rays.y required");
+    if (!out) return fail(ctx, BHX_E_INVALID, "out is NULL");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    rc = ensure_catalogue(ctx, s);
+    if (rc) return rc;
+    ShadeArgs P = host::shade_args(*s);
+    P.bb = ctx->d_bb;
+    P.cell_start = static_cast<const uint32_t*>(ctx->cell_start.p);
+    P.star = static_cast<const double*>(ctx->star_rows.p);
+    if (s->stars_enabled && !P.cell_start) return fail(ctx, BHX_E_INVALID, "no star catalogue");
+    P.cls = rays->cls;
+    P.y = rays->y;
+    P.jet = rays->jet;
+    P.xyz = out->xyz;
+    P.ray_xyz = out->ray_xyz;
+    P.g = out->g;
+    P.dir = out->dir;
+    const int blocks = (P.n_pix + 255) / 256;
+    HIPCHK(ctx->flux_partial.ensure(sizeof(double) * blocks));
+    P.flux_partial = static_cast<double*>(ctx->flux_partial.p);
+    P.class_counts = ctx->d_cnt->classes;
+    HIPCHK(launch_zero_u64(ctx->d_cnt->classes, 8, st));
+    HIPCHK(hipEventRecord(ctx->ev[2], st));
+    HIPCHK(launch_shade(P, blocks, st));
+    HIPCHK(hipEventRecord(ctx->ev[3], st));
+    if (stats) {
+        HIPCHK(hipStreamSynchronize(st));
+        Counters c;
+        HIPCHK(hipMemcpy(&c, ctx->d_cnt, sizeof c, hipMemcpyDeviceToHost));
+        std::vector<double> fp(blocks);
+        HIPCHK(hipMemcpy(fp.data(), P.flux_partial, sizeof(double) * blocks, hipMemcpyDeviceToHost));
+        double flux = 0.0;
+        for (double v : fp) flux += v;
+        stats->captured += static_cast<int64_t>(c.classes[BHX_HORIZON]);
+        stats->disc_hits += static_cast<int64_t>(c.classes[BHX_DISC]);
+        stats->sun_hits += static_cast<int64_t>(c.classes[BHX_SUN]);
+        stats->star_hits += static_cast<int64_t>(c.classes[BHX_STAR]);
+        stats->escaped += static_cast<int64_t>(c.classes[BHX_SKY]);
+        stats->exhausted += static_cast<int64_t>(c.classes[BHX_EXHAUSTED]);
+        stats->jet_rays += static_cast<int64_t>(c.classes[6]);
+        stats->total_flux += flux;
+        stats->shade_ms = ms_between(ctx->ev[2], ctx->ev[3]);
+    }
+    return BHX_OK;
+}
+
+int bhx_render(bhx_context* ctx, const bhx_scene* s, double* xyz, bhx_stats* stats) {
+    if (!ctx) return BHX_E_INVALID;
+    int rc = check_scene(ctx, s);
+    if (rc) return rc;
+    if (!xyz) return fail(ctx, BHX_E_INVALID, "xyz is NULL");
+    const auto t0 = std::chrono::steady_clock::now();
+    HIPCHK(hipSetDevice(ctx->device));
+    const int S = std::max(1, s->sqrt_spp);
+    const size_t n_rays = static_cast<size_t>(s->width) * s->height * S * S;
+    const size_t n_pix = static_cast<size_t>(s->width) * s->height;
+    HIPCHK(ctx->r_cls.ensure(n_rays));
+    HIPCHK(ctx->r_y.ensure(8 * sizeof(double) * n_rays));
+    HIPCHK(ctx->r_jet.ensure(sizeof(double) * n_rays));
+    HIPCHK(ctx->r_xyz.ensure(3 * sizeof(double) * n_pix));
+    bhx_rays rays{};
+    rays.cls = static_cast<uint8_t*>(ctx->r_cls.p);
+    rays.y = static_cast<double*>(ctx->r_y.p);
+    rays.jet = static_cast<double*>(ctx->r_jet.p);
+    bhx_shaded sh{};
+    sh.xyz = static_cast<double*>(ctx->r_xyz.p);
+    bhx_stats local;
+    std::memset(&local, 0, sizeof local);
+    rc = bhx_trace_dev(ctx, s, &rays, &local, nullptr);
+    if (rc) return rc;
+    rc = bhx_shade_dev(ctx, s, &rays, &sh, &local, nullptr);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(xyz, sh.xyz, 3 * sizeof(double) * n_pix, hipMemcpyDeviceToHost));
+    if (stats) {
+        stats->rays += local.rays;
+        stats->steps += local.steps;
+        stats->captured += local.captured;
+        stats->disc_hits += local.disc_hits;
+        stats->sun_hits += local.sun_hits;
+        stats->star_hits += local.star_hits;
+        stats->jet_rays += local.jet_rays;
+        stats->escaped += local.escaped;
+        stats->exhausted += local.exhausted;
+        stats->total_flux += local.total_flux;
+        stats->max_norm_error = std::max(stats->max_norm_error, local.max_norm_error);
+        stats->max_carter_drift = std::max(stats->max_carter_drift, local.max_carter_drift);
+        stats->attempts += local.attempts;
+        stats->rhs_evals += local.rhs_evals;
+        stats->trace_ms = local.trace_ms;
+        stats->shade_ms = local.shade_ms;
+        stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return BHX_OK;
+}
+
+int bhx_auto_exposure_dev(bhx_context* ctx, const double* xyz, int64_t n, double key, double* exposure,
+                          void* stream) {
+    if (!ctx) return BHX_E_INVALID;
+    if (!xyz || !exposure || n < 0) return fail(ctx, BHX_E_INVALID, "bad argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    HIPCHK(ctx->lum_hist.ensure(sizeof(unsigned int) * 65536));
+    HIPCHK(ctx->lum_npos.ensure(sizeof(unsigned long long)));
+    unsigned int* hist = static_cast<unsigned int*>(ctx->lum_hist.p);
+    unsigned long long* npos = static_cast<unsigned long long*>(ctx->lum_npos.p);
+    // count positive finite luminances with the first pass of the first selection
+    std::vector<unsigned int> h(65536);
+    unsigned long long n_pos = 0;
+    auto select = [&](uint64_t k, double* out) -> int {
+        unsigned long long prefix = 0;
+        for (int pass = 0; pass < 4; ++pass) {
+            const int shift = 48 - 16 * pass;
+            HIPCHK(launch_zero_u32(hist, 65536, st));
+            if (pass == 0) HIPCHK(launch_zero_u64(npos, 1, st));
+            HIPCHK(launch_lum_hist(xyz, n, shift, prefix, hist, pass == 0 ? npos : nullptr, st));
+            HIPCHK(hipMemcpyAsync(h.data(), hist, sizeof(unsigned int) * 65536, hipMemcpyDeviceToHost, st));
+            if (pass == 0) HIPCHK(hipMemcpyAsync(&n_pos, npos, sizeof n_pos, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            if (pass == 0 && k == UINT64_MAX) return 1;  // count-only call
+            uint64_t cum = 0;
+            int digit = 0;
+            for (; digit < 65536; ++digit) {
+                if (cum + h[digit] > k) break;
+                cum += h[digit];
+            }
+            if (digit == 65536) return fail(ctx, BHX_E_HIP, "radix select lost the k-th element");
+            k -= cum;
+            prefix = (prefix << 16) | static_cast<unsigned long long>(digit);
+        }
+        std::memcpy(out, &prefix, 8);
+        return 0;
+    };
+    int rc = select(UINT64_MAX, nullptr);
+    if (rc < 0) return rc;
+    if (n_pos == 0) {
+        *exposure = 1.0;
+        return BHX_OK;
+    }
+    // image.cpp:29-44: k = floor(q (n-1)) order statistics, q = 0.999 and 0.9999
+    auto kth = [&](double q) {
+        uint64_t k = static_cast<uint64_t>(q * static_cast<double>(n_pos - 1));
+        return std::min<uint64_t>(k, n_pos - 1);
+    };
+    double hi = 0.0, peak = 0.0;
+    rc = select(kth(0.999), &hi);
+    if (rc) return rc;
+    rc = select(kth(0.9999), &peak);
+    if (rc) return rc;
+    double e = 1.0;
+    if (hi > 0.0) e = key / hi;
+    if (peak > 0.0) e = std::min(e, 3.0 * key / peak);
+    *exposure = e;
+    return BHX_OK;
+}
+
+int bhx_develop_dev(bhx_context* ctx, const double* xyz, int64_t n, double exposure, int32_t tm,
+                    double log_decades, uint8_t* rgb, void* stream) {
+    if (!ctx) return BHX_E_INVALID;
+    if (!xyz || !rgb || n < 0) return fail(ctx, BHX_E_INVALID, "bad argument");
+    if (tm < BHX_TM_LINEAR || tm > BHX_TM_HDR) return fail(ctx, BHX_E_INVALID, "unknown tone map");
+    HIPCHK(hipSetDevice(ctx->device));
+    const double log_k = std::pow(10.0, std::clamp(log_decades, 1.0, 20.0));
+    if (n == 0) return BHX_OK;
+    HIPCHK(launch_develop(xyz, n, exposure, tm, log_k, rgb, static_cast<hipStream_t>(stream)));
+    return BHX_OK;
+}
+
+int bhx_blackbody_xyz_dev(bhx_context* ctx, const double* T, int64_t n, double* xyz, void* stream) {
+    if (!ctx) return BHX_E_INVALID;
+    if (!T || !xyz || n < 0) return fail(ctx, BHX_E_INVALID, "bad argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(launch_bb(ctx->d_bb, T, n, xyz, static_cast<hipStream_t>(stream)));
+    return BHX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,46 @@
+// host_setup.h - host-side scene preparation for the HIP tracer (C++17).
+//
+// The per-frame scalar work that the reference does on the CPU before its
+// thread pool starts (src/render.cpp:399-418) and the one-off host solvers
+// its CLI calls (src/main.cpp:399-423 setup_jet, :472-522 setup_neutron_star):
+// spectrum table, star catalogue and its cell index, disc constants, camera
+// tetrad, jet normalisation, TOV structure.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "params.h"
+#include "../../include/bhx.h"
+
+namespace bhx {
+namespace host {
+
+// spec:: table of natural-log X, Y, Z over log10 T in [log10 50, log10 3e9]
+// (src/spectrum.cpp:97-114): 3 x 4096 doubles, component-major.
+const std::vector<double>& bb_table();
+void blackbody_xyz(double T, double out[3]);     // src/spectrum.cpp:123-134
+void powerlaw_xyz(double alpha, double out[3]);  // src/spectrum.cpp:77-92
+
+// A star catalogue, unit directions + colour temperature + irradiance.
+struct Catalogue {
+    std::vector<double> x, y, z, T, irr;
+};
+// The reference's StarField(count, seed) generator (src/scene.h:62-107).
+void generate_catalogue(int count, uint64_t seed, Catalogue& cat);
+// Cell index on 360 x 720 lat-lon bins (src/scene.h:173-181): stars sorted by
+// cell, ascending index within a cell; rows of 6 doubles x y z irr X/Y Z/Y.
+void bin_catalogue(const Catalogue& cat, std::vector<uint32_t>& cell_start,
+                   std::vector<double>& rows);
+
+// Scene -> kernel arguments.  Pointers and queue fields are left for the caller.
+TraceArgs trace_args(const bhx_scene& s);
+ShadeArgs shade_args(const bhx_scene& s);
+
+// Kerr helpers used by the set-up code.
+double isco_radius(double a, int sense);
+double horizon_outer(double a);
+double horizon_inner(double a);
+
+}  // namespace host
+}  // namespace bhx

@@ -1,0 +1,58 @@
+"""ctypes binding of libbhx.so (the HIP tracer's C ABI, include/bhx.h).
+
+There is no CPU fallback: if the library is missing or cannot be loaded this
+module raises, and every entry point that needs the GPU fails loudly.
+"""
+import ctypes as C
+import os
+
+from .abi import Launch, Rays, Scene, Shaded, Stats
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libbhx.so")
+
+_lib = None
+
+P = C.c_void_p
+SYMBOLS = (
+    "bhx_create", "bhx_destroy", "bhx_last_error", "bhx_abi_version", "bhx_set_catalogue",
+    "bhx_render", "bhx_trace_dev", "bhx_shade_dev", "bhx_auto_exposure_dev", "bhx_develop_dev",
+    "bhx_blackbody_xyz_dev", "bhx_set_launch", "bhx_get_launch",
+)
+
+
+class BhxError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libbhx.so once (no GPU call is made by loading)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise BhxError(f"HIP library not built: {LIB_PATH} (run python -c "
+                       f"'import __graft_entry__ as g; g.build()')")
+    L = C.CDLL(LIB_PATH)
+    L.bhx_create.argtypes = [C.c_int, C.POINTER(P)]
+    L.bhx_destroy.argtypes = [P]
+    L.bhx_last_error.argtypes = [P]
+    L.bhx_last_error.restype = C.c_char_p
+    L.bhx_abi_version.restype = C.c_int
+    L.bhx_set_catalogue.argtypes = [P, C.c_int32, P, P, P, P, P]
+    L.bhx_render.argtypes = [P, C.POINTER(Scene), P, C.POINTER(Stats)]
+    L.bhx_trace_dev.argtypes = [P, C.POINTER(Scene), C.POINTER(Rays), C.POINTER(Stats), P]
+    L.bhx_shade_dev.argtypes = [P, C.POINTER(Scene), C.POINTER(Rays), C.POINTER(Shaded),
+                                C.POINTER(Stats), P]
+    L.bhx_auto_exposure_dev.argtypes = [P, P, C.c_int64, C.c_double, C.POINTER(C.c_double), P]
+    L.bhx_develop_dev.argtypes = [P, P, C.c_int64, C.c_double, C.c_int32, C.c_double, P, P]
+    L.bhx_blackbody_xyz_dev.argtypes = [P, P, C.c_int64, P, P]
+    L.bhx_set_launch.argtypes = [P, C.POINTER(Launch)]
+    L.bhx_get_launch.argtypes = [P, C.POINTER(Launch)]
+    _lib = L
+    return L
+
+
+def check(ctx, rc, what):
+    if rc != 0:
+        msg = lib().bhx_last_error(ctx)
+        raise BhxError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")

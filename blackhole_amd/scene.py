@@ -1,0 +1,96 @@
+"""Scene construction: RenderConfig defaults and the CLI's scene-setup rules.
+
+Mirrors the reference's configuration layer so that a scene is described the
+way a user of the reference describes it:
+  * RenderConfig defaults (src/render.h:15-53), Camera (src/scene.h:343-351),
+    Jet defaults (src/jet.h:105-145), NeutronStar defaults (src/neutronstar.h:43-67),
+    HotSpot defaults (src/scene.h:306-313);
+  * the inclination clamp to [0.5, 179.5] degrees (src/main.cpp:810);
+  * the escape-radius rule r_escape = max(3 r_cam + 100, ...) (src/main.cpp:882-890).
+The numerical host set-up (Jet::configure, TOV) is native code in the C ABI
+library (blackhole_amd/csrc/host_setup.cpp).
+"""
+import math
+
+from .abi import Scene
+
+M_SUN = 1.98847e30
+
+
+def default_scene():
+    """A bhx_scene holding the reference's RenderConfig defaults."""
+    s = Scene()
+    s.M_kg = 10.0 * M_SUN
+    s.spin = 0.9
+    s.cam_r, s.cam_theta, s.cam_phi, s.cam_fov = 60.0, math.pi / 2.0, 0.0, 0.6
+    s.cam_yaw = s.cam_pitch = s.cam_roll = 0.0
+    s.width, s.height, s.sqrt_spp = 1280, 720, 2
+    s.disc_enabled, s.disc_r_out, s.eddington, s.disc_sense = 1, 24.0, 0.1, 1
+    s.stars_enabled, s.star_count, s.seed = 1, 250000, 20240921
+    s.sun_enabled = 0
+    s.sun_radius, s.sun_T_eff = 1.0, 5772.0
+    s.hs_enabled, s.hs_r, s.hs_phi0, s.hs_sigma, s.hs_contrast, s.hs_omega, s.hs_sense = (
+        0, 8.0, 0.0, 0.6, 25.0, 0.0, 1)
+    s.ns_enabled = 0
+    s.ns_M_kg, s.ns_R_m, s.ns_R, s.ns_T_eff = 1.4 * M_SUN, 12000.0, 5.8, 1.0e6
+    s.ns_spin_hz, s.ns_Omega = 0.0, 0.0
+    s.ns_caps, s.ns_cap_tilt, s.ns_cap_radius, s.ns_cap_T = 0, math.radians(60.0), math.radians(20.0), 3.0e6
+    s.ns_cap_phase0, s.ns_two_caps = 0.0, 1
+    s.jet_enabled = 0
+    s.jet_z_base, s.jet_z_top, s.jet_R_base, s.jet_k = 3.0, 120.0, 1.6, 0.58
+    s.jet_gamma_max, s.jet_z_sat, s.jet_alpha = 10.0, 80.0, 0.7
+    s.jet_u_peak, s.jet_u_wid, s.jet_spine, s.jet_spine_wid, s.jet_zeta = 0.82, 0.17, 0.06, 0.40, 2.0
+    s.jet_a = s.jet_Omega_F = s.jet_j_nu0 = s.jet_r_bound = 0.0
+    s.observer_time = 0.0
+    s.rtol, s.atol, s.r_escape, s.max_steps = 1e-9, 1e-11, 4000.0, 20000
+    s.threads = 0
+    return s
+
+
+def inclination(deg):
+    """Camera theta for an inclination in degrees (src/main.cpp:810)."""
+    return min(max(deg, 0.5), 179.5) * math.pi / 180.0
+
+
+def escape_radius(s):
+    """r_escape as the CLI sets it (src/main.cpp:882-890)."""
+    far = s.cam_r * 3.0 + 100.0
+    if s.sun_enabled:
+        sep = math.sqrt(sum(c * c for c in s.sun_centre))
+        far = max(far, (sep + s.sun_radius) * 3.0)
+    if s.jet_enabled:
+        far = max(far, s.jet_r_bound * 1.5)
+    return far
+
+
+def plain_scene(*, spin, incl_deg, r_cam, width, height, fov, disc_r_out, eddington,
+                stars=False, star_count=250000, rtol=1e-9, atol=1e-11, max_steps=20000,
+                sqrt_spp=1, M_kg=10.0 * M_SUN, seed=20240921):
+    s = default_scene()
+    s.M_kg = M_kg
+    s.spin = spin
+    s.cam_r = r_cam
+    s.cam_theta = inclination(incl_deg)
+    s.cam_fov = fov
+    s.width, s.height, s.sqrt_spp = width, height, sqrt_spp
+    s.disc_enabled, s.disc_r_out, s.eddington = 1, disc_r_out, eddington
+    s.stars_enabled, s.star_count = int(stars), star_count
+    s.seed = seed
+    s.rtol, s.atol, s.max_steps = rtol, atol, max_steps
+    s.r_escape = escape_radius(s)
+    return s
+
+
+def config1(width=256, height=256):
+    """Plain FP64 render: a=0.9, 10 Msun, r=500, 80 deg, +-20M image plane,
+    NT disc ISCO-20M at 0.1 Eddington, no stars, rtol 1e-10 / atol 1e-12."""
+    return plain_scene(spin=0.9, incl_deg=80.0, r_cam=500.0, width=width, height=height,
+                       fov=2.0 * math.atan(20.0 / 500.0), disc_r_out=20.0, eddington=0.1,
+                       rtol=1e-10, atol=1e-12)
+
+
+def config2(width=1024, height=1024, stars=True):
+    """Plain FP64 render at 1024^2: a=0.998, 60 deg, r=1000, NT disc ISCO-50M at
+    0.3 Eddington, 250k-star sky (+-55M image plane, 10 Msun)."""
+    return plain_scene(spin=0.998, incl_deg=60.0, r_cam=1000.0, width=width, height=height,
+                       fov=2.0 * math.atan(0.055), disc_r_out=50.0, eddington=0.3, stars=stars)

@@ -1,0 +1,170 @@
+"""Host-side mirror of the reference's frame API over the HIP tracer.
+
+`Tracer.render(scene)` is the Python face of bh::render (src/render.h:72):
+it returns the absolute CIE XYZ frame and the accumulated RenderStats.  The
+stage calls (trace / shade / auto_exposure / develop) keep everything in
+device memory; torch provides the allocations and the stream, the arithmetic
+is all in libbhx.so.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import native
+from .abi import TM_HDR, Launch, Rays, Scene, Shaded, Stats
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+class Tracer:
+    """One bhx_context on one GPU."""
+
+    def __init__(self, device=0):
+        import torch
+        self.torch = torch
+        if not torch.cuda.is_available():
+            raise native.BhxError("no GPU visible: the HIP tracer has no CPU path")
+        self.device = device
+        self.lib = native.lib()
+        self.ctx = C.c_void_p()
+        torch.cuda.set_device(device)
+        native.check(None, self.lib.bhx_create(device, C.byref(self.ctx)), "bhx_create")
+
+    def close(self):
+        if self.ctx:
+            self.lib.bhx_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _stream(self, stream):
+        s = stream if stream is not None else self.torch.cuda.current_stream(self.device)
+        return C.c_void_p(s.cuda_stream)
+
+    # ---- configuration -----------------------------------------------------
+    def set_catalogue(self, x=None, y=None, z=None, T=None, irr=None):
+        """Load an explicit star catalogue (host arrays); no arguments = the
+        reference generator StarField(star_count, seed)."""
+        if x is None:
+            native.check(self.ctx, self.lib.bhx_set_catalogue(self.ctx, 0, None, None, None, None, None),
+                         "bhx_set_catalogue")
+            return
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z, T, irr)]
+        native.check(self.ctx, self.lib.bhx_set_catalogue(
+            self.ctx, len(arrs[0]), *[a.ctypes.data_as(C.c_void_p) for a in arrs]), "bhx_set_catalogue")
+
+    def set_launch(self, block=0, waves_per_simd=0, chunk=0, refill=0):
+        native.check(self.ctx, self.lib.bhx_set_launch(
+            self.ctx, C.byref(Launch(block, waves_per_simd, chunk, refill))), "bhx_set_launch")
+
+    def get_launch(self):
+        l = Launch()
+        native.check(self.ctx, self.lib.bhx_get_launch(self.ctx, C.byref(l)), "bhx_get_launch")
+        return {"block": l.block, "waves_per_simd": l.waves_per_simd, "chunk": l.chunk,
+                "refill": l.refill}
+
+    # ---- drop-in frame API ---------------------------------------------------
+    def render(self, scene):
+        """bh::render: (xyz[H, W, 3] float64 numpy, Stats)."""
+        xyz = np.zeros((scene.height, scene.width, 3))
+        st = Stats()
+        native.check(self.ctx, self.lib.bhx_render(self.ctx, C.byref(scene),
+                                                   xyz.ctypes.data_as(C.c_void_p), C.byref(st)),
+                     "bhx_render")
+        return xyz, st
+
+    # ---- device stages -------------------------------------------------------
+    def alloc_rays(self, scene, full=True):
+        t = self.torch
+        n = scene.n_rays
+        dev = f"cuda:{self.device}"
+        out = {"cls": t.empty(n, dtype=t.uint8, device=dev),
+               "y": t.empty((8, n), dtype=t.float64, device=dev),
+               "jet": t.empty(n, dtype=t.float64, device=dev)}
+        if full:
+            out.update(steps=t.empty(n, dtype=t.int32, device=dev),
+                       attempts=t.empty(n, dtype=t.int32, device=dev),
+                       norm_err=t.empty(n, dtype=t.float64, device=dev),
+                       carter=t.empty(n, dtype=t.float64, device=dev))
+        return out
+
+    @staticmethod
+    def _rays_struct(r):
+        return Rays(*[(r[k].data_ptr() if r.get(k) is not None else None)
+                      for k in ("cls", "y", "jet", "steps", "attempts", "norm_err", "carter")])
+
+    def trace(self, scene, rays=None, stats=None, stream=None):
+        """trace() of every sample into device arrays; returns (rays, stats)."""
+        if rays is None:
+            rays = self.alloc_rays(scene)
+        st = stats if stats is not None else Stats()
+        native.check(self.ctx, self.lib.bhx_trace_dev(self.ctx, C.byref(scene),
+                                                      C.byref(self._rays_struct(rays)), C.byref(st),
+                                                      self._stream(stream)), "bhx_trace_dev")
+        return rays, st
+
+    def trace_async(self, scene, rays, stream=None):
+        """Stream-ordered trace without a host synchronisation (for timing loops)."""
+        native.check(self.ctx, self.lib.bhx_trace_dev(self.ctx, C.byref(scene),
+                                                      C.byref(self._rays_struct(rays)), None,
+                                                      self._stream(stream)), "bhx_trace_dev")
+
+    def shade(self, scene, rays, out=None, stats=None, stream=None, per_ray=True):
+        t = self.torch
+        dev = f"cuda:{self.device}"
+        if out is None:
+            out = {"xyz": t.empty((scene.height, scene.width, 3), dtype=t.float64, device=dev)}
+            if per_ray:
+                n = scene.n_rays
+                out.update(ray_xyz=t.empty((3, n), dtype=t.float64, device=dev),
+                           g=t.empty(n, dtype=t.float64, device=dev),
+                           dir=t.empty((3, n), dtype=t.float64, device=dev))
+        st = stats if stats is not None else Stats()
+        sh = Shaded(*[(out[k].data_ptr() if out.get(k) is not None else None)
+                      for k in ("xyz", "ray_xyz", "g", "dir")])
+        native.check(self.ctx, self.lib.bhx_shade_dev(self.ctx, C.byref(scene),
+                                                      C.byref(self._rays_struct(rays)), C.byref(sh),
+                                                      C.byref(st), self._stream(stream)), "bhx_shade_dev")
+        return out, st
+
+    def shade_async(self, scene, rays, out, stream=None):
+        sh = Shaded(*[(out[k].data_ptr() if out.get(k) is not None else None)
+                      for k in ("xyz", "ray_xyz", "g", "dir")])
+        native.check(self.ctx, self.lib.bhx_shade_dev(self.ctx, C.byref(scene),
+                                                      C.byref(self._rays_struct(rays)), C.byref(sh),
+                                                      None, self._stream(stream)), "bhx_shade_dev")
+
+    def auto_exposure(self, xyz, key=1.0, stream=None):
+        e = C.c_double()
+        n = xyz.numel() // 3
+        native.check(self.ctx, self.lib.bhx_auto_exposure_dev(self.ctx, _ptr(xyz), n, key, C.byref(e),
+                                                              self._stream(stream)), "bhx_auto_exposure_dev")
+        return e.value
+
+    def develop(self, xyz, exposure, tonemap=TM_HDR, log_decades=16.0, stream=None):
+        t = self.torch
+        n = xyz.numel() // 3
+        rgb = t.empty(n * 3, dtype=t.uint8, device=xyz.device)
+        native.check(self.ctx, self.lib.bhx_develop_dev(self.ctx, _ptr(xyz), n, exposure, tonemap,
+                                                        log_decades, _ptr(rgb), self._stream(stream)),
+                     "bhx_develop_dev")
+        return rgb.view(-1, 3)
+
+    def blackbody_xyz(self, T, stream=None):
+        t = self.torch
+        out = t.empty((T.numel(), 3), dtype=t.float64, device=T.device)
+        native.check(self.ctx, self.lib.bhx_blackbody_xyz_dev(self.ctx, _ptr(T), T.numel(), _ptr(out),
+                                                              self._stream(stream)), "bhx_blackbody_xyz_dev")
+        return out

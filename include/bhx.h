@@ -1,0 +1,211 @@
+/*
+ * bhx.h - C ABI of the MI355X (gfx950) Kerr geodesic tracer.
+ *
+ * This is the drop-in boundary for the reference's frame API
+ *     img::Image bh::render(const RenderConfig&, RenderStats&)      (src/render.h:72)
+ * whose body (src/render.cpp:396-497) spawns a std::thread row pool and calls
+ * the internal per-ray functions trace() (src/render.cpp:69-260), shade_hit()
+ * (src/render.cpp:265-375) and shade() (src/render.cpp:382-393).  Those three
+ * have internal linkage, so the only seam a maintainer can replace is the
+ * thread pool inside render(): the entry points below are what such a
+ * replacement calls.  The C++ wrapper that keeps the reference's own
+ * signature lives in blackhole_amd/csrc/host/render.h.
+ *
+ * Conventions
+ *   - Plain C: no C++ or torch types, fixed-size integer types, FP64 data.
+ *   - Every function returns 0 on success or a negative BHX_E_* code; the
+ *     message of the last failure on a context is bhx_last_error(ctx).
+ *   - Functions whose name ends in _dev take DEVICE pointers and a hipStream_t
+ *     passed as void* (NULL = the null stream); they are stream-ordered and do
+ *     not synchronise unless stated.  All other pointers are host pointers.
+ *   - Ray index of sample (sx, sy) of pixel (x, y):
+ *         ray = (y * width + x) * S*S + sy * S + sx,     S = sqrt_spp
+ *     which is the order the reference visits samples (src/render.cpp:437-459).
+ *   - SoA ray arrays are component-major: component c of ray i is at
+ *     [c * n_rays + i].
+ */
+#ifndef BHX_H
+#define BHX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BHX_ABI_VERSION 1
+
+/* Error codes. */
+#define BHX_OK 0
+#define BHX_E_INVALID (-1)    /* bad argument (NULL pointer, size, range)   */
+#define BHX_E_HIP (-2)        /* a HIP runtime call failed                  */
+#define BHX_E_NOMEM (-3)      /* device allocation failed                   */
+#define BHX_E_TOO_LARGE (-4)  /* request exceeds a documented maximum       */
+
+/* Termination class of a ray: the order of Hit::What (src/render.cpp:37). */
+enum bhx_class {
+    BHX_HORIZON = 0,
+    BHX_DISC = 1,
+    BHX_SUN = 2,
+    BHX_STAR = 3,       /* neutron-star surface */
+    BHX_SKY = 4,
+    BHX_EXHAUSTED = 5
+};
+
+/* Tone curves of img::ToneMap (src/image.h:16). */
+enum bhx_tonemap {
+    BHX_TM_LINEAR = 0,
+    BHX_TM_REINHARD = 1,
+    BHX_TM_ACES = 2,
+    BHX_TM_LOG = 3,
+    BHX_TM_HDR = 4
+};
+
+/*
+ * One scene: RenderConfig (src/render.h:15-53) flattened into POD, including
+ * the members of Camera (src/scene.h:343-351), SunBody (src/scene.h:232-235),
+ * HotSpot (src/scene.h:306-313), NeutronStar (src/neutronstar.h:43-67) and
+ * Jet (src/jet.h:96-145, the values after Jet::configure, src/jet.h:261-311).
+ * Units are the reference's: lengths in GM/c^2, angles in radians.
+ */
+typedef struct bhx_scene {
+    double M_kg;              /* RenderConfig::M_kg                         */
+    double spin;              /* RenderConfig::spin                         */
+    double cam_r, cam_theta, cam_phi, cam_fov, cam_yaw, cam_pitch, cam_roll;
+    double disc_r_out;        /* RenderConfig::disc_r_out                   */
+    double eddington;         /* RenderConfig::eddington                    */
+    double observer_time;     /* RenderConfig::observer_time                */
+    double rtol, atol;        /* RenderConfig::rtol / atol                  */
+    double r_escape;          /* RenderConfig::r_escape                     */
+    /* SunBody */
+    double sun_centre[3], sun_radius, sun_T_eff;
+    /* HotSpot */
+    double hs_r, hs_phi0, hs_sigma, hs_contrast, hs_omega;
+    /* NeutronStar */
+    double ns_M_kg, ns_R_m, ns_R, ns_T_eff, ns_spin_hz, ns_Omega;
+    double ns_cap_tilt, ns_cap_radius, ns_cap_T, ns_cap_phase0;
+    /* Jet (configured) */
+    double jet_z_base, jet_z_top, jet_R_base, jet_k, jet_gamma_max, jet_z_sat;
+    double jet_alpha, jet_u_peak, jet_u_wid, jet_spine, jet_spine_wid, jet_zeta;
+    double jet_a, jet_Omega_F, jet_j_nu0, jet_r_bound;
+    uint64_t seed;            /* RenderConfig::seed (jitter + star catalogue) */
+    int32_t width, height, sqrt_spp, max_steps;
+    int32_t disc_enabled, disc_sense, stars_enabled, star_count;
+    int32_t sun_enabled, hs_enabled, hs_sense, ns_enabled;
+    int32_t ns_caps, ns_two_caps, jet_enabled, threads;
+} bhx_scene;
+
+/* RenderStats (src/render.h:55-69) plus the counters the reference lacks. */
+typedef struct bhx_stats {
+    int64_t rays, steps, captured, disc_hits, sun_hits, star_hits, jet_rays;
+    int64_t escaped, exhausted;
+    double max_norm_error, max_carter_drift, seconds, total_flux;
+    int64_t attempts;         /* DP5 attempts, accepted + rejected           */
+    int64_t rhs_evals;        /* geodesic_rhs evaluations on the GPU          */
+    double trace_ms;          /* device time of the trace kernel (HIP events) */
+    double shade_ms;          /* device time of the shading kernel            */
+} bhx_stats;
+
+/* Per-ray outputs of the tracer (device pointers; any may be NULL). */
+typedef struct bhx_rays {
+    uint8_t *cls;             /* n_rays, enum bhx_class                       */
+    double *y;                /* 8 x n_rays SoA: state Hit::y at termination  */
+    double *jet;              /* n_rays: Hit::jet                             */
+    int32_t *steps;           /* n_rays: accepted DP5 steps                   */
+    int32_t *attempts;        /* n_rays: DP5 attempts incl. rejected          */
+    double *norm_err;         /* n_rays: max |g^ab p_a p_b| / |terms| on ray  */
+    double *carter;           /* n_rays: max |Q - Q0| / (|Q0| + 1e-12)        */
+} bhx_rays;
+
+/* Per-ray shading outputs (device pointers; any may be NULL). */
+typedef struct bhx_shaded {
+    double *xyz;              /* n_pixels x 3, interleaved like img::Image    */
+    double *ray_xyz;          /* 3 x n_rays SoA: colour of each sample        */
+    double *g;                /* n_rays: redshift factor (disc / NS), else 0  */
+    double *dir;              /* 3 x n_rays SoA: escape direction (sky), else 0 */
+} bhx_shaded;
+
+typedef struct bhx_context bhx_context;
+
+/* ---- context ---------------------------------------------------------- */
+
+/* Opens device `device` and allocates the context's tables and counters. */
+int bhx_create(int device, bhx_context **out);
+int bhx_destroy(bhx_context *ctx);
+const char *bhx_last_error(const bhx_context *ctx);
+int bhx_abi_version(void);
+
+/*
+ * Star catalogue.  By default a scene with stars_enabled uses the reference's
+ * generator, StarField(star_count, seed) (src/scene.h:62-107), built on the
+ * host and binned on 360 x 720 cells (src/scene.h:173-181).  This call loads
+ * an arbitrary catalogue instead (host arrays, n stars, unit directions,
+ * colour temperature in K, irradiance in W m^-2); n = 0 returns to the
+ * generator.  The context keeps its own copy.
+ */
+int bhx_set_catalogue(bhx_context *ctx, int32_t n, const double *x, const double *y,
+                      const double *z, const double *temperature, const double *irradiance);
+
+/* ---- frame API --------------------------------------------------------- */
+
+/*
+ * Drop-in for bh::render (src/render.h:72; body src/render.cpp:396-497):
+ * traces and shades every sample of the scene on the GPU, averages the
+ * samples of each pixel, and writes width*height*3 doubles of absolute CIE
+ * XYZ radiance (row-major, like img::Image) to the HOST buffer `xyz`.
+ * `stats` is accumulated like RenderStats (counters +=, errors max,
+ * seconds assigned).  Synchronous.
+ */
+int bhx_render(bhx_context *ctx, const bhx_scene *scene, double *xyz, bhx_stats *stats);
+
+/*
+ * Tracer stage alone: trace() (src/render.cpp:69-260) for every sample of the
+ * scene, one persistent launch over an atomic work queue.  Output arrays hold
+ * n_rays = width*height*sqrt_spp^2 entries indexed by ray.  `stats` may be
+ * NULL; when given, the call synchronises the stream and accumulates rays,
+ * steps, attempts, RHS evaluations and the invariant maxima (the per-class
+ * counters are filled by bhx_shade_dev).
+ */
+int bhx_trace_dev(bhx_context *ctx, const bhx_scene *scene, const bhx_rays *rays,
+                  bhx_stats *stats, void *stream);
+
+/*
+ * Shading stage: shade() (src/render.cpp:382-393) of every ray of the frame
+ * from the tracer's outputs (cls, y, jet over ALL rays of the scene, ray_begin
+ * 0), then the per-pixel average of src/render.cpp:460-462.
+ */
+int bhx_shade_dev(bhx_context *ctx, const bhx_scene *scene, const bhx_rays *rays,
+                  const bhx_shaded *out, bhx_stats *stats, void *stream);
+
+/* img::auto_exposure (src/image.cpp:22-45) on a device XYZ frame; the
+ * result is written to the host double *exposure.  Synchronous. */
+int bhx_auto_exposure_dev(bhx_context *ctx, const double *xyz, int64_t n_pixels, double key,
+                          double *exposure, void *stream);
+
+/* img::develop (src/image.cpp:119-191): device XYZ frame -> device sRGB8. */
+int bhx_develop_dev(bhx_context *ctx, const double *xyz, int64_t n_pixels, double exposure,
+                    int32_t tonemap, double log_decades, uint8_t *rgb, void *stream);
+
+/* spec::blackbody_xyz (src/spectrum.cpp:123-134) of n temperatures, device
+ * arrays, through the same 4096-entry table the shader uses. */
+int bhx_blackbody_xyz_dev(bhx_context *ctx, const double *T, int64_t n, double *xyz,
+                          void *stream);
+
+/* ---- tuning ------------------------------------------------------------ */
+
+/* Launch shape of the persistent tracer.  0 in a field keeps the default. */
+typedef struct bhx_launch {
+    int32_t block;            /* threads per workgroup (multiple of 64)       */
+    int32_t waves_per_simd;   /* resident waves per SIMD the grid is sized for */
+    int32_t chunk;            /* rays a wave takes from the queue at once     */
+    int32_t refill;           /* refill when at least this many lanes idle    */
+} bhx_launch;
+
+int bhx_set_launch(bhx_context *ctx, const bhx_launch *launch);
+int bhx_get_launch(const bhx_context *ctx, bhx_launch *launch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BHX_H */

@@ -1,0 +1,250 @@
+"""CPU oracle of the per-pixel Kerr tracer -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
+this package, and only as the checker or the timed CPU baseline; the product
+(blackhole_amd/) never imports it.
+
+Two libraries are wrapped:
+  lib/liboracle.so  the repository's own CPU restatement (src/oracle.cpp) of the
+                    reference's algorithm, bitwise-pinned to the reference;
+  _ref/libbhref.so  the reference itself, compiled from its sources by
+                    oracle/Makefile where /root/reference exists (optional).
+
+Parity status per function: every function of src/oracle.cpp is pinned, see
+the header of src/oracle.cpp and tests/test_oracle_ref.py / test_oracle_golden.py.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from blackhole_amd.abi import Scene, Stats
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "lib", "liboracle.so")
+REF = os.path.join(HERE, "_ref", "libbhref.so")
+
+_lib = None
+_ref = None
+
+P = C.c_void_p
+D = C.c_double
+DP = C.POINTER(C.c_double)
+
+
+class Catalogue(C.Structure):
+    _fields_ = [("n", C.c_int32), ("x", P), ("y", P), ("z", P), ("temperature", P),
+                ("irradiance", P)]
+
+
+class OracleRays(C.Structure):
+    _fields_ = [("cls", P), ("y", P), ("jet", P), ("steps", P), ("attempts", P),
+                ("norm_err", P), ("carter", P), ("g", P), ("dir", P), ("ray_xyz", P)]
+
+
+def build():
+    """Compile lib/liboracle.so (and _ref/libbhref.so when the reference is present)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"oracle library missing: {LIB} (run make -C oracle)")
+        _lib = C.CDLL(LIB)
+        _lib.oracle_auto_exposure.restype = C.c_double
+        _lib.oracle_isco.restype = C.c_double
+        _lib.oracle_page_thorne_W.restype = C.c_double
+        _lib.oracle_disc_temperature.restype = C.c_double
+        _lib.oracle_isco.argtypes = [D, C.c_int]
+        _lib.oracle_page_thorne_W.argtypes = [D, D, D]
+        _lib.oracle_auto_exposure.argtypes = [P, C.c_int64, D]
+        _lib.oracle_develop.argtypes = [P, C.c_int64, D, C.c_int, D, P]
+        _lib.oracle_blackbody_xyz.argtypes = [D, P]
+        _lib.oracle_blackbody_xyz_exact.argtypes = [D, P]
+        _lib.oracle_powerlaw_xyz.argtypes = [D, P]
+        _lib.oracle_integrate.argtypes = [D, P, D, D, D, C.c_int, D, D, P]
+        _lib.oracle_geodesic_rhs.argtypes = [D, P, P]
+        _lib.oracle_make_photon.argtypes = [D, D, D, D, P, P]
+        _lib.oracle_invariants.argtypes = [D, P, P, P, P, P, P]
+        _lib.oracle_star_radiance.argtypes = [P, P, P, D, P]
+        _lib.oracle_generate_catalogue.argtypes = [C.c_int32, C.c_uint64, P, P, P, P, P]
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(REF)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF):
+            raise RuntimeError(f"reference shim missing: {REF}")
+        _ref = C.CDLL(REF)
+        _ref.bhref_auto_exposure.restype = C.c_double
+        _ref.bhref_auto_exposure.argtypes = [P, C.c_int64, D]
+        _ref.bhref_develop.argtypes = [P, C.c_int64, D, C.c_int, D, P]
+        _ref.bhref_blackbody_xyz.argtypes = [D, P]
+        _ref.bhref_star_radiance.argtypes = [C.c_int32, C.c_uint64, P, D, P]
+        _ref.bhref_polytrope_star.argtypes = [D, D, D, P, P]
+        _ref.bhref_polytrope_solve.argtypes = [D, D, D, P, P]
+    return _ref
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(P)
+
+
+class Cat:
+    """Holds numpy arrays alive for a Catalogue struct."""
+
+    def __init__(self, x, y, z, T, irr):
+        self.arrays = [np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, z, T, irr)]
+        self.struct = Catalogue(len(self.arrays[0]), *[_ptr(v) for v in self.arrays])
+
+    @property
+    def ref(self):
+        return C.byref(self.struct)
+
+
+def generate_catalogue(count, seed):
+    """The reference's StarField(count, seed) catalogue (scene.h:62-107)."""
+    arrs = [np.empty(count) for _ in range(5)]
+    lib().oracle_generate_catalogue(count, seed, *[_ptr(a) for a in arrs])
+    return arrs
+
+
+def render(scene, cat=None, threads=0):
+    """Whole-frame render (CPU).  Returns (xyz[H, W, 3], Stats)."""
+    xyz = np.zeros((scene.height, scene.width, 3))
+    st = Stats()
+    rc = lib().oracle_render(C.byref(scene), cat.ref if cat else None, _ptr(xyz), C.byref(st),
+                             threads)
+    if rc:
+        raise RuntimeError(f"oracle_render failed: {rc}")
+    return xyz, st
+
+
+def trace_list(scene, ray_ids, cat=None, threads=0):
+    """Trace + shade explicit rays.  Returns a dict of per-ray numpy arrays."""
+    ids = np.ascontiguousarray(ray_ids, dtype=np.int64)
+    n = len(ids)
+    out = {
+        "cls": np.zeros(n, np.uint8), "y": np.zeros((8, n)), "jet": np.zeros(n),
+        "steps": np.zeros(n, np.int32), "attempts": np.zeros(n, np.int32),
+        "norm_err": np.zeros(n), "carter": np.zeros(n), "g": np.zeros(n),
+        "dir": np.zeros((3, n)), "ray_xyz": np.zeros((3, n)),
+    }
+    s = OracleRays(*[_ptr(out[k]) for k in ("cls", "y", "jet", "steps", "attempts", "norm_err",
+                                            "carter", "g", "dir", "ray_xyz")])
+    rc = lib().oracle_trace_list(C.byref(scene), cat.ref if cat else None, _ptr(ids),
+                                 C.c_int64(n), C.byref(s), threads)
+    if rc:
+        raise RuntimeError(f"oracle_trace_list failed: {rc}")
+    return out
+
+
+def _rays_out(n):
+    return {
+        "cls": np.zeros(n, np.uint8), "y": np.zeros((8, n)), "jet": np.zeros(n),
+        "steps": np.zeros(n, np.int32), "attempts": np.zeros(n, np.int32),
+        "norm_err": np.zeros(n), "carter": np.zeros(n), "g": np.zeros(n),
+        "dir": np.zeros((3, n)), "ray_xyz": np.zeros((3, n)),
+    }
+
+
+def _rays_struct(out):
+    return OracleRays(*[_ptr(out[k]) for k in ("cls", "y", "jet", "steps", "attempts", "norm_err",
+                                                "carter", "g", "dir", "ray_xyz")])
+
+
+def trace_uv(scene, u, v, cat=None, threads=0):
+    """Trace + shade rays from explicit image-plane coordinates."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    out = _rays_out(len(u))
+    rc = lib().oracle_trace_uv(C.byref(scene), cat.ref if cat else None, _ptr(u), _ptr(v),
+                               C.c_int64(len(u)), C.byref(_rays_struct(out)), threads)
+    if rc:
+        raise RuntimeError(f"oracle_trace_uv failed: {rc}")
+    return out
+
+
+def ray_uv(scene, ray_ids):
+    ids = np.ascontiguousarray(ray_ids, dtype=np.int64)
+    u = np.zeros(len(ids))
+    v = np.zeros(len(ids))
+    lib().oracle_ray_uv(C.byref(scene), _ptr(ids), C.c_int64(len(ids)), _ptr(u), _ptr(v))
+    return u, v
+
+
+def ref_render(scene):
+    xyz = np.zeros((scene.height, scene.width, 3))
+    st = Stats()
+    ref().bhref_render(C.byref(scene), _ptr(xyz), C.byref(st))
+    return xyz, st
+
+
+def ref_trace_list(scene, ray_ids):
+    ids = np.ascontiguousarray(ray_ids, dtype=np.int64)
+    n = len(ids)
+    out = {"cls": np.zeros(n, np.uint8), "y": np.zeros((8, n)), "jet": np.zeros(n),
+           "steps": np.zeros(n, np.int32), "g": np.zeros(n), "ray_xyz": np.zeros((3, n))}
+    ref().bhref_trace_list(C.byref(scene), _ptr(ids), C.c_int64(n),
+                           *[_ptr(out[k]) for k in ("cls", "y", "jet", "steps", "g", "ray_xyz")])
+    return out
+
+
+def auto_exposure(xyz, key=1.0):
+    x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    return lib().oracle_auto_exposure(_ptr(x), len(x), key)
+
+
+def develop(xyz, exposure, tonemap=4, log_decades=16.0):
+    x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    rgb = np.zeros((len(x), 3), np.uint8)
+    lib().oracle_develop(_ptr(x), len(x), exposure, tonemap, log_decades, _ptr(rgb))
+    return rgb
+
+
+def blackbody_xyz(T):
+    o = np.zeros(3)
+    lib().oracle_blackbody_xyz(T, _ptr(o))
+    return o
+
+
+def blackbody_xyz_exact(T):
+    o = np.zeros(3)
+    lib().oracle_blackbody_xyz_exact(T, _ptr(o))
+    return o
+
+
+def geodesic_rhs(a, y):
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    d = np.zeros(8)
+    lib().oracle_geodesic_rhs(a, _ptr(y), _ptr(d))
+    return d
+
+
+def make_photon(a, r, th, phi, direction):
+    d = np.ascontiguousarray(direction, dtype=np.float64)
+    y = np.zeros(8)
+    lib().oracle_make_photon(a, r, th, phi, _ptr(d), _ptr(y))
+    return y
+
+
+def invariants(a, y):
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    vals = [C.c_double() for _ in range(5)]
+    lib().oracle_invariants(a, _ptr(y), *[C.byref(v) for v in vals])
+    return dict(zip(("E", "Lz", "Q", "norm", "norm_rel"), [v.value for v in vals]))
+
+
+def integrate(a, y, h0, rtol, atol, max_steps, r_min, r_max):
+    y = np.array(y, dtype=np.float64)
+    n = C.c_int64(0)
+    rc = lib().oracle_integrate(a, _ptr(y), h0, rtol, atol, max_steps, r_min, r_max, C.byref(n))
+    return y, rc, n.value

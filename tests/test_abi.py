@@ -1,0 +1,58 @@
+"""The C ABI: libbhx.so loads without a GPU, exports every symbol that
+include/bhx.h declares, and the ctypes mirror has the C struct layouts."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from blackhole_amd import abi, native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "bhx.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(bhx_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = native.lib()
+    names = declared_functions()
+    assert len(names) >= 12
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(native.SYMBOLS) <= set(names)
+    assert lib.bhx_abi_version() == 1
+
+
+def test_no_gpu_call_needed_for_errors():
+    lib = native.lib()
+    # NULL context / arguments are rejected before any HIP call
+    assert lib.bhx_create(0, None) == -1
+    assert lib.bhx_render(None, None, None, None) == -1
+    assert lib.bhx_destroy(None) == 0
+
+
+@pytest.mark.parametrize("struct,cname", [(abi.Scene, "bhx_scene"), (abi.Stats, "bhx_stats"),
+                                          (abi.Rays, "bhx_rays"), (abi.Shaded, "bhx_shaded"),
+                                          (abi.Launch, "bhx_launch")])
+def test_struct_layout_matches_header(tmp_path, struct, cname):
+    fields = [f[0] for f in struct._fields_]
+    src = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{HEADER}"', "int main(void){",
+           f'printf("%zu\\n", sizeof({cname}));']
+    for f in fields:
+        src.append(f'printf("%zu\\n", offsetof({cname}, {f}));')
+    src.append("return 0;}")
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", str(c), "-o", str(exe)], check=True)
+    vals = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True,
+                                           check=True).stdout.split()]
+    assert vals[0] == C.sizeof(struct)
+    for f, off in zip(fields, vals[1:]):
+        assert getattr(struct, f).offset == off, f
