@@ -14,6 +14,7 @@
 #include <vector>
 
 namespace bhx {
+hipError_t launch_raygen(const RaygenArgs& args, hipStream_t stream);
 hipError_t launch_trace(const TraceArgs& args, int grid, int block, hipStream_t stream);
 hipError_t trace_occupancy(int block, int* blocks_per_cu);
 hipError_t launch_shade(const ShadeArgs& args, int blocks, hipStream_t s);
@@ -70,10 +71,12 @@ struct bhx_context {
     uint64_t cat_seed = 0;
     bool cat_is_custom = false;
     DevBuf cell_start, star_rows;
-    // tile order cache
-    int tiles_w = -1, tiles_h = -1;
-    DevBuf tile_order;
+    // queue order cache: tiles centre-first, first queue slot of each tile
+    int tiles_w = -1, tiles_h = -1, tiles_S = -1;
+    DevBuf tile_order, tile_start;
     int n_tiles = 0;
+    // initial ray states in queue order (kInitRows x n) and slot -> ray map
+    DevBuf init, q2ray;
     // counters
     Counters* d_cnt = nullptr;
     DevBuf flux_partial, lum_hist, lum_npos;
@@ -108,9 +111,9 @@ int check_scene(bhx_context* ctx, const bhx_scene* s) {
 
 // Centre-first order of 8 x 8 tiles: the photon ring (longest rays) sits near
 // the optical axis, so starting there shortens the queue's tail.
-int ensure_tiles(bhx_context* ctx, int W, int H) {
+int ensure_tiles(bhx_context* ctx, int W, int H, int S) {
     const int tw = (W + 7) / 8, th = (H + 7) / 8;
-    if (tw == ctx->tiles_w && th == ctx->tiles_h) return BHX_OK;
+    if (tw == ctx->tiles_w && th == ctx->tiles_h && S == ctx->tiles_S) return BHX_OK;
     const int n = tw * th;
     std::vector<int32_t> order(n);
     std::vector<double> key(n);
@@ -120,10 +123,18 @@ int ensure_tiles(bhx_context* ctx, int W, int H) {
         key[i] = cx * cx + cy * cy;
     }
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+    std::vector<int32_t> start(n + 1, 0);
+    for (int t = 0; t < n; ++t) {
+        const int tx = order[t] % tw, ty = order[t] / tw;
+        start[t + 1] = start[t] + std::min(8, W - tx * 8) * std::min(8, H - ty * 8) * S * S;
+    }
     HIPCHK(ctx->tile_order.ensure(sizeof(int32_t) * n));
     HIPCHK(hipMemcpy(ctx->tile_order.p, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    HIPCHK(ctx->tile_start.ensure(sizeof(int32_t) * (n + 1)));
+    HIPCHK(hipMemcpy(ctx->tile_start.p, start.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice));
     ctx->tiles_w = tw;
     ctx->tiles_h = th;
+    ctx->tiles_S = S;
     ctx->n_tiles = n;
     return BHX_OK;
 }
@@ -220,7 +231,8 @@ int bhx_destroy(bhx_context* ctx) {
     (void)hipDeviceSynchronize();
     if (ctx->d_bb) (void)hipFree(ctx->d_bb);
     if (ctx->d_cnt) (void)hipFree(ctx->d_cnt);
-    for (DevBuf* b : {&ctx->cell_start, &ctx->star_rows, &ctx->tile_order, &ctx->flux_partial,
+    for (DevBuf* b : {&ctx->cell_start, &ctx->star_rows, &ctx->tile_order, &ctx->tile_start,
+                      &ctx->init, &ctx->q2ray, &ctx->flux_partial,
                       &ctx->lum_hist, &ctx->lum_npos, &ctx->r_cls, &ctx->r_y, &ctx->r_jet, &ctx->r_xyz})
         b->release();
     for (auto& e : ctx->ev)
@@ -278,15 +290,25 @@ int bhx_trace_dev(bhx_context* ctx, const bhx_scene* s, const bhx_rays* rays, bh
     if (!rays) return fail(ctx, BHX_E_INVALID, "rays is NULL");
     HIPCHK(hipSetDevice(ctx->device));
     hipStream_t st = static_cast<hipStream_t>(stream);
-    rc = ensure_tiles(ctx, s->width, s->height);
+    const int Sspp = std::max(1, s->sqrt_spp);
+    rc = ensure_tiles(ctx, s->width, s->height, Sspp);
     if (rc) return rc;
+    const int64_t n_rays = static_cast<int64_t>(s->width) * s->height * Sspp * Sspp;
+    HIPCHK(ctx->init.ensure(sizeof(double) * kInitRows * n_rays));
+    HIPCHK(ctx->q2ray.ensure(sizeof(int32_t) * n_rays));
+    RaygenArgs G = host::raygen_args(*s);
+    G.tiles_x = ctx->tiles_w;
+    G.n_tiles = ctx->n_tiles;
+    G.tile_order = static_cast<const int32_t*>(ctx->tile_order.p);
+    G.tile_start = static_cast<const int32_t*>(ctx->tile_start.p);
+    G.init = static_cast<double*>(ctx->init.p);
+    G.q2ray = static_cast<int32_t*>(ctx->q2ray.p);
     TraceArgs P = host::trace_args(*s);
-    P.tiles_x = ctx->tiles_w;
-    P.n_tiles = ctx->n_tiles;
     P.refill = std::max(1, std::min(64, ctx->launch.refill));
-    P.tile_order = static_cast<const int32_t*>(ctx->tile_order.p);
-    P.tile_counter = &ctx->d_cnt->tile_counter;
-    P.n_rays = static_cast<int64_t>(s->width) * s->height * P.S * P.S;
+    P.chunk = std::max(1, ctx->launch.chunk);
+    P.init = G.init;
+    P.q2ray = G.q2ray;
+    P.counter = &ctx->d_cnt->tile_counter;
     P.cls = rays->cls;
     P.y = rays->y;
     P.jet = rays->jet;
@@ -300,6 +322,7 @@ int bhx_trace_dev(bhx_context* ctx, const bhx_scene* s, const bhx_rays* rays, bh
     if (rc) return rc;
     // counters are cleared by a fill kernel (never a memset node: the call may be captured)
     HIPCHK(launch_zero_u64(reinterpret_cast<unsigned long long*>(ctx->d_cnt), sizeof(Counters) / 8, st));
+    HIPCHK(launch_raygen(G, st));
     HIPCHK(hipEventRecord(ctx->ev[0], st));
     HIPCHK(launch_trace(P, grid, block, st));
     HIPCHK(hipEventRecord(ctx->ev[1], st));
@@ -310,10 +333,10 @@ int bhx_trace_dev(bhx_context* ctx, const bhx_scene* s, const bhx_rays* rays, bh
         double mn, mc;
         std::memcpy(&mn, &c.totals[2], 8);
         std::memcpy(&mc, &c.totals[3], 8);
-        stats->rays += P.n_rays;
+        stats->rays += n_rays;
         stats->steps += static_cast<int64_t>(c.totals[0]);
         stats->attempts += static_cast<int64_t>(c.totals[1]);
-        stats->rhs_evals += static_cast<int64_t>(c.totals[1]) * 6 + P.n_rays;
+        stats->rhs_evals += static_cast<int64_t>(c.totals[1]) * 6 + n_rays;
         stats->max_norm_error = std::max(stats->max_norm_error, mn);
         stats->max_carter_drift = std::max(stats->max_carter_drift, mc);
         stats->trace_ms = ms_between(ctx->ev[0], ctx->ev[1]);

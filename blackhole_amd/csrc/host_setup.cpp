@@ -204,18 +204,11 @@ void bin_catalogue(const Catalogue& cat, std::vector<uint32_t>& cell_start, std:
     }
 }
 
-TraceArgs trace_args(const bhx_scene& s) {
-    TraceArgs P{};
+RaygenArgs raygen_args(const bhx_scene& s) {
+    RaygenArgs P{};
     const double a = s.spin;
     P.a = a;
     P.a2 = a * a;
-    P.r_h = horizon_outer(a);
-    P.r_stop = P.r_h * 1.0 + 1e-4 + (P.r_h - horizon_inner(a)) * 1e-3;  // render.cpp:72-73
-    P.rtol = s.rtol;
-    P.atol = s.atol;
-    P.r_escape = s.r_escape;
-    P.h0 = 0.02 * std::max(1.0, s.cam_r);                              // render.cpp:86
-    P.pw_init = std::pow(std::max(1e-4, 1e-4), 0.4 / 5.0);             // geodesic.h:171-172
     // camera point: Geom, ZAMO tetrad, covariant metric (kerr.h:44-77, 270-293)
     const double r = s.cam_r, th = s.cam_theta;
     double sn = std::sin(th);
@@ -255,8 +248,24 @@ TraceArgs trace_args(const bhx_scene& s) {
     P.W = s.width;
     P.H = s.height;
     P.S = Sspp;
-    P.max_steps = s.max_steps;
     P.seed = s.seed;
+    P.n_rays = s.width * s.height * Sspp * Sspp;
+    return P;
+}
+
+TraceArgs trace_args(const bhx_scene& s) {
+    TraceArgs P{};
+    const double a = s.spin;
+    P.a = a;
+    P.a2 = a * a;
+    P.r_h = horizon_outer(a);
+    P.r_stop = P.r_h * 1.0 + 1e-4 + (P.r_h - horizon_inner(a)) * 1e-3;  // render.cpp:72-73
+    P.rtol = s.rtol;
+    P.atol = s.atol;
+    P.r_escape = s.r_escape;
+    P.h0 = 0.02 * std::max(1.0, s.cam_r);                              // render.cpp:86
+    P.pw_init = std::pow(std::max(1e-4, 1e-4), 0.4 / 5.0);             // geodesic.h:171-172
+    P.max_steps = s.max_steps;
     P.has_disc = s.disc_enabled != 0;
     if (P.has_disc) {  // NovikovThorneDisc ctor (disc.h:73-82)
         const double ac = std::clamp(s.spin, -0.9999, 0.9999);
@@ -265,6 +274,8 @@ TraceArgs trace_args(const bhx_scene& s) {
     }
     P.has_ns = s.ns_enabled != 0;
     P.ns_R = s.ns_R;
+    const int Sspp = std::max(1, s.sqrt_spp);
+    P.n_rays = s.width * s.height * Sspp * Sspp;
     return P;
 }
 

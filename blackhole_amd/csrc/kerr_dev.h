@@ -62,6 +62,48 @@ __device__ __forceinline__ void sincos_near(double x, double* s, double* c) {
     *c = ((q + 1) & 2) ? -b : b;
 }
 
+// exp(x) for |x| < 700 to <= 1 ulp (checked against glibc): Cody-Waite
+// reduction by ln 2 (fdlibm's two-part constant), degree-13 Taylor-Horner on
+// |r| <= ln(2)/2 (remainder < 4e-18), scale by 2^k.
+__device__ __forceinline__ double exp_near(double x) {
+    const double k = rint(x * 1.4426950408889634);
+    double r = fma(-k, 6.93147180369123816490e-01, x);
+    r = fma(-k, 1.90821492927058770002e-10, r);
+    double p = 1.0 / 6227020800.0;
+    p = fma(p, r, 1.0 / 479001600.0);
+    p = fma(p, r, 1.0 / 39916800.0);
+    p = fma(p, r, 1.0 / 3628800.0);
+    p = fma(p, r, 1.0 / 362880.0);
+    p = fma(p, r, 1.0 / 40320.0);
+    p = fma(p, r, 1.0 / 5040.0);
+    p = fma(p, r, 1.0 / 720.0);
+    p = fma(p, r, 1.0 / 120.0);
+    p = fma(p, r, 1.0 / 24.0);
+    p = fma(p, r, 1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return ldexp(p, static_cast<int>(k));
+}
+
+// log(x) for finite normal x > 0 to <= 1 ulp (checked against glibc): the
+// reduction and minimax kernel of fdlibm's __ieee754_log (Sun, 1993).
+__device__ __forceinline__ double log_pos(double x) {
+    int e;
+    double m = frexp(x, &e);
+    if (m < 0.70710678118654752440) { m *= 2.0; e -= 1; }
+    const double f = m - 1.0;
+    const double s = f * rcp(2.0 + f);
+    const double z = s * s, w = z * z;
+    const double t1 = w * (3.999999999940941908e-01 + w * (2.222219843214978396e-01 + w * 1.531383769920937332e-01));
+    const double t2 = z * (6.666666666666735130e-01 + w * (2.857142874366239149e-01 +
+                           w * (1.818357216161805012e-01 + w * 1.479819860511658591e-01)));
+    const double R = t2 + t1;
+    const double hfsq = 0.5 * f * f;
+    const double dk = e;
+    return dk * 6.93147180369123816490e-01 - ((hfsq - (s * (hfsq + R) + dk * 1.90821492927058770002e-10)) - f);
+}
+
 __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }  // std::max
 __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }  // std::min
 __device__ __forceinline__ double dclamp(double v, double lo, double hi) {             // std::clamp

@@ -5,14 +5,11 @@
 
 namespace bhx {
 
-// Everything the tracer needs about one scene, precomputed on the host
-// (camera tetrad, metric at the camera, horizon radii, disc edges).  Passed by
-// value as a kernel argument, so every field lands in scalar registers.
-struct TraceArgs {
-    // metric and integrator (src/render.cpp:71-86, src/render.h:45-48)
-    double a, a2, r_h, r_stop, rtol, atol, r_escape, h0;
-    double pw_init;                 // pow(1e-4, 0.08): controller memory of a fresh ray
-    // camera (src/scene.h:343-385, src/kerr.h:283-293, src/geodesic.h:215-231)
+// Ray generation: camera, jitter and the initial photon of every sample
+// (src/render.cpp:437-452, src/scene.h:356-384, src/geodesic.h:215-231),
+// written in work-queue order so the tracer's refills are coalesced loads.
+struct RaygenArgs {
+    double a, a2;
     double cam_r, cam_th, cam_ph;
     double e00, e03, e11, e22, e33;           // ZAMO tetrad, non-zero entries
     double gl_tt, gl_tp, gl_rr, gl_hh, gl_pp; // g_{mu nu} at the camera
@@ -20,17 +17,34 @@ struct TraceArgs {
     double tanf, aspect, inv_S;
     double c_roll, s_roll, c_pitch, s_pitch, c_yaw, s_yaw;
     int32_t has_roll;
-    int32_t W, H, S, max_steps;
+    int32_t W, H, S;
     uint64_t seed;
+    int32_t tiles_x, n_tiles;
+    const int32_t* tile_order;                // queue tile -> image tile (8 x 8 pixels)
+    const int32_t* tile_start;                // queue tile -> first queue slot
+    int32_t n_rays;
+    double* init;                             // kInitRows x n_rays, queue order
+    int32_t* q2ray;                           // queue slot -> ray index
+};
+
+// Rows of the initial-state array.
+enum InitRow { kI_T = 0, kI_R, kI_TH, kI_PH, kI_PR, kI_PTH, kI_PT, kI_PPH,
+               kI_K1 = 8, kI_C0 = 14, kI_Q0 = 15, kInitRows = 16 };
+
+// The tracer's loop: only what the integration and the events need, so
+// every uniform value stays in scalar registers.
+struct TraceArgs {
+    // metric and integrator (src/render.cpp:71-86, src/render.h:45-48)
+    double a, a2, r_h, r_stop, rtol, atol, r_escape, h0;
+    double pw_init;                 // pow(1e-4, 0.08): controller memory of a fresh ray
     // events (src/render.cpp:158-199)
-    int32_t has_disc, has_ns;
     double r_in, r_out, ns_R;
-    // work queue: tiles of 8 x 8 pixels in the order given by tile_order
-    int32_t tiles_x, n_tiles, refill;
-    const int32_t* tile_order;
-    uint32_t* tile_counter;
+    int32_t has_disc, has_ns, max_steps, refill, chunk;
+    int32_t n_rays;
+    const double* init;             // kInitRows x n_rays, queue order
+    const int32_t* q2ray;
+    uint32_t* counter;              // next queue slot
     // outputs (SoA over ray index, see bhx.h)
-    int64_t n_rays;
     uint8_t* cls;
     double* y;
     double* jet;
