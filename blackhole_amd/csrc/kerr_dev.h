@@ -110,73 +110,73 @@ __device__ __forceinline__ double dclamp(double v, double lo, double hi) {      
     return (v < lo) ? lo : ((hi < v) ? hi : v);
 }
 
-// Metric at a point plus the partials the RHS needs.
+// Per-ray constants of the momentum equations (p_t, p_phi are conserved).
+struct RayK {
+    double a, a2, pt, pph;
+    double apt, apph, a2pt2, pph2;   // a p_t, a p_phi, a^2 p_t^2, p_phi^2
+};
+__device__ __forceinline__ RayK ray_k(double a, double a2, double pt, double pph) {
+    RayK k;
+    k.a = a; k.a2 = a2; k.pt = pt; k.pph = pph;
+    k.apt = a * pt; k.apph = a * pph; k.a2pt2 = a2 * pt * pt; k.pph2 = pph * pph;
+    return k;
+}
+
+// Right-hand side plus the geometry the per-step invariants reuse.
 struct Rhs {
-    double d[6];                       // dt, dr, dtheta, dphi, dp_r, dp_theta
-    double cs, sn2, isn2;              // cos(theta), clamped sin^2, 1/sin^2
-    double gtt, gtp, grr, ghh, gpp;    // g^{mu nu}
+    double d[6];                    // dt, dr, dtheta, dphi, dp_r, dp_theta
+    double cs, sn2, isn2;           // cos(theta), clamped sin^2, 1/sin^2
+    double iS, iD, D, rpa;          // 1/Sigma, 1/Delta, Delta, r^2 + a^2
 };
 
 // Hamilton's equations dx/dl = g^{mu nu} p_nu, dp/dl = -1/2 d(g^{ab}) p_a p_b
-// (src/geodesic.h:50-71) with the partials of src/kerr.h:205-252.
-__device__ __forceinline__ void kerr_rhs(double a, double a2, double r, double th, double pt,
-                                         double pr, double pth, double pph, Rhs& o) {
+// (src/geodesic.h:50-71) for the Kerr inverse metric of src/kerr.h:107-176,
+// written through the separable form of the contraction
+//     Sigma g^{ab} p_a p_b = Delta p_r^2 + p_th^2 + U^2 / sin^2 - P^2 / Delta,
+//     P = (r^2 + a^2) p_t + a p_phi,   U = p_phi + a p_t sin^2,
+// whose r and theta partials are
+//     d_r  = [Delta' p_r^2 - 4 r p_t P / Delta + P^2 Delta' / Delta^2 - 2 r H] / Sigma,
+//     d_th = 2 sin cos [a^2 p_t^2 - p_phi^2 / sin^4 + a^2 H] / Sigma,
+// with H = g^{ab} p_a p_b.  Algebraically identical to the component-wise
+// partials of the reference (checked against the oracle's RHS to rounding),
+// about 40% fewer operations, and a single reciprocal of Sigma Delta sin^2.
+__device__ __forceinline__ void kerr_rhs(const RayK& K, double r, double th, double pr, double pth,
+                                         Rhs& o) {
     double sn, cs;
     sincos_near(th, &sn, &cs);
     if (fabs(sn) < 1e-9) sn = (sn < 0.0) ? -1e-9 : 1e-9;  // kerr.h:34,51
-    const double sn2 = sn * sn, cs2 = cs * cs;
+    const double sn2 = sn * sn;
     const double r2 = r * r;
-    const double rpa = r2 + a2;
-    const double S = r2 + a2 * cs2;
-    const double D = r2 - 2.0 * r + a2;
-    const double A = rpa * rpa - a2 * D * sn2;
-    const double iS = rcp(S), iD = rcp(D), isn2 = rcp(sn2);
-    const double iQ = iS * iD;                   // 1 / (Sigma Delta)
-    // g^{mu nu}
-    const double gtt = -A * iQ;
-    const double gtp = -2.0 * a * r * iQ;
-    const double grr = D * iS;
-    const double ghh = iS;
-    const double P = D - a2 * sn2;
-    const double gpp = P * iQ * isn2;
-    // partials of the building blocks
-    const double snc = sn * cs;
-    const double Sr = 2.0 * r, Sh = -2.0 * a2 * snc;
+    const double rpa = r2 + K.a2;
+    const double S = r2 + K.a2 * (cs * cs);
+    const double D = rpa - 2.0 * r;
+    const double SD = S * D;
+    const double iX = rcp(SD * sn2);          // 1 / (Sigma Delta sin^2)
+    const double iS = D * sn2 * iX;
+    const double iD = S * sn2 * iX;
+    const double isn2 = SD * iX;
+    const double P = rpa * K.pt + K.apph;
+    const double PiD = P * iD;
+    const double U = K.pph + K.apt * sn2;
+    const double pr2 = pr * pr;
+    const double N = D * pr2 + pth * pth + U * U * isn2 - P * PiD;
+    const double H = N * iS;
     const double Dr = 2.0 * r - 2.0;
-    const double Ar = 4.0 * r * rpa - a2 * Dr * sn2;
-    const double Ah = -2.0 * a2 * D * snc;
-    const double Q = S * D;
-    const double Qr = Sr * D + S * Dr;
-    const double Qh = Sh * D;
-    const double iQ2 = iQ * iQ, iS2 = iS * iS;
-    const double dr_tt = -(Ar * Q - A * Qr) * iQ2;
-    const double dh_tt = -(Ah * Q - A * Qh) * iQ2;
-    const double N = -2.0 * a * r;
-    const double dr_tp = (-2.0 * a * Q - N * Qr) * iQ2;
-    const double dh_tp = -N * Qh * iQ2;
-    const double dr_rr = (Dr * S - D * Sr) * iS2;
-    const double dh_rr = -D * Sh * iS2;
-    const double dr_hh = -Sr * iS2;
-    const double dh_hh = -Sh * iS2;
-    const double R = Q * sn2;
-    const double Rr = Qr * sn2;
-    const double Rh = Qh * sn2 + Q * 2.0 * snc;
-    const double iR = iQ * isn2;
-    const double iR2 = iR * iR;
-    const double dr_pp = (Dr * R - P * Rr) * iR2;
-    const double dh_pp = (Sh * R - P * Rh) * iR2;
-    const double tptp = 2.0 * pt * pph;
-    const double ptpt = pt * pt, pp2 = pph * pph, pr2 = pr * pr, ph2 = pth * pth;
-    o.d[0] = gtt * pt + gtp * pph;
-    o.d[1] = grr * pr;
-    o.d[2] = ghh * pth;
-    o.d[3] = gtp * pt + gpp * pph;
-    o.d[4] = -0.5 * (dr_tt * ptpt + dr_tp * tptp + dr_rr * pr2 + dr_hh * ph2 + dr_pp * pp2);
-    o.d[5] = -0.5 * (dh_tt * ptpt + dh_tp * tptp + dh_rr * pr2 + dh_hh * ph2 + dh_pp * pp2);
+    const double br = Dr * pr2 + PiD * (Dr * PiD - 4.0 * r * K.pt) - 2.0 * r * H;
+    const double bt = K.a2pt2 + K.a2 * H - K.pph2 * isn2 * isn2;
+    o.d[0] = (K.a * U - rpa * PiD) * iS;
+    o.d[1] = D * pr * iS;
+    o.d[2] = pth * iS;
+    o.d[3] = (K.apt + K.pph * isn2 - K.a * PiD) * iS;
+    o.d[4] = -0.5 * iS * br;
+    o.d[5] = -(sn * cs) * iS * bt;
     o.cs = cs;
     o.sn2 = sn2;
     o.isn2 = isn2;
-    o.gtt = gtt; o.gtp = gtp; o.grr = grr; o.ghh = ghh; o.gpp = gpp;
+    o.iS = iS;
+    o.iD = iD;
+    o.D = D;
+    o.rpa = rpa;
 }
 
 // splitmix64 (the reference's Rng, src/scene.h:27-44), counter form: draw n

@@ -200,7 +200,7 @@ __global__ void __launch_bounds__(256) k_shade(const ShadeArgs P) {
             } else if (cls == BHX_SKY) {
                 // escape direction (render.cpp:15-26, 358-368)
                 Rhs f;
-                kerr_rhs(P.a, P.a2, y[1], y[2], y[4], y[5], y[6], y[7], f);
+                kerr_rhs(ray_k(P.a, P.a2, y[4], y[7]), y[1], y[2], y[5], y[6], f);
                 const double r = y[1];
                 const double R = sqrt(r * r + P.a2);
                 double sn, cs, sp, cp;
