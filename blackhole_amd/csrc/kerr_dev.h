@@ -2,14 +2,14 @@
 //
 // Boyer-Lindquist Kerr metric, its closed-form r/theta partials and the
 // Hamiltonian right-hand side, restructured for CDNA4's FP64 VALU:
-//   * the 15 divisions of the textbook form (src/kerr.h:107-176) become
-//     three Newton-refined reciprocals (1/Sigma, 1/Delta, 1/sin^2 theta)
-//     shared by every component;
+//   * the 15 divisions of the textbook form (src/kerr.h:107-176) become one
+//     Newton-refined reciprocal of Sigma Delta sin^2 theta shared by every
+//     component (separable form of the contraction, see kerr_rhs);
 //   * p_t and p_phi are constants of the motion (their RHS is exactly 0,
 //     src/geodesic.h:64-65), so a ray carries them as scalars and integrates
 //     only the six components (t, r, theta, phi, p_r, p_theta);
-//   * one sincos per evaluation; the geometry of the last stage (FSAL) is
-//     reused for the per-step invariants and the equatorial sign test.
+//   * one branch-free sincos per evaluation; the geometry of the last stage
+//     (FSAL) is reused for the per-step invariants and the equatorial sign test.
 // Results agree with the reference expressions to a few ulp; the rounding
 // differences are far below the integrator's own error estimate noise.
 #pragma once
@@ -33,14 +33,11 @@ __device__ __forceinline__ double rcp(double x) {
 // sin and cos of |x| <= 1e5 to <= 1 ulp (checked against glibc): Cody-Waite
 // reduction by pi/2 with a two-term split evaluated by FMA, then the minimax
 // kernels of Sun's fdlibm (__kernel_sin / __kernel_cos, 1993) on
-// [-pi/4, pi/4].  About 35 VALU ops and few registers, against ~70 for the
-// library routine whose Payne-Hanek branch is inlined into every RHS copy.
-// Larger |x| (a ray that wound round the pole thousands of times) falls back
-// to the library routine, out of line.
-__device__ __noinline__ void sincos_far(double x, double* s, double* c) { sincos(x, s, c); }
-
+// [-pi/4, pi/4].  About 35 VALU ops, few registers and no branch, against ~70
+// for the library routine whose Payne-Hanek branch would split every RHS into
+// separate basic blocks.  theta cannot leave that range: each DP5 step moves it
+// by less than pi and a ray takes at most 20000 steps.  NaN stays NaN.
 __device__ __forceinline__ void sincos_near(double x, double* s, double* c) {
-    if (!(fabs(x) <= 1.0e5)) { sincos_far(x, s, c); return; }
     const double k = rint(x * 0.63661977236758134308);
     double r = fma(-k, 1.5707963267948966, x);
     r = fma(-k, 6.123233995736766e-17, r);

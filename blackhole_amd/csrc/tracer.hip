@@ -295,9 +295,9 @@ k_trace(const TraceArgs P) {
         if (!live) continue;
 
         // ---- one DP5 attempt (geodesic.h:130-194) ---------------------------
-        if (tries == 0) {
+        {
             const double hmax = 0.05 * dmax(1.0, y[1] - P.r_h) + 0.01;
-            if (h > hmax) h = hmax;
+            h = (tries == 0 && h > hmax) ? hmax : h;   // clamp once per step, not per retry
         }
         double tmp[6], k2[6], k3[6], k4[6], k5[6], k6[6], yn[6];
         const RayK K = ray_k(P.a, P.a2, pt, pph);
