@@ -1,0 +1,155 @@
+"""Write the golden fixtures of tests/golden/ from the reference build.
+
+Runs the reference tracer compiled from its own sources (oracle/_ref/
+libbhref.so, see oracle/Makefile) on small scenes and stores its outputs:
+per-ray class, hit state, jet integral, step count, redshift and colour on a
+subset of rays, the SHA-256 of the whole XYZ frame, RenderStats, plus
+colour-table, star-field, display, TOV and jet-normalisation samples.
+Inputs and expected outputs only; nothing of the reference's source.
+
+    python tests/golden/make_golden.py
+"""
+import ctypes as C
+import hashlib
+import json
+import math
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from blackhole_amd import scene as S  # noqa: E402
+from blackhole_amd.abi import Scene  # noqa: E402
+
+M_SUN = S.M_SUN
+G, C_LIGHT = 6.67430e-11, 2.99792458e8
+
+
+def scene_dict(sc):
+    out = {}
+    for name, typ in Scene._fields_:
+        v = getattr(sc, name)
+        out[name] = list(v) if name == "sun_centre" else v
+    return out
+
+
+def scenes():
+    out = {}
+    out["config1_64"] = S.config1(64, 64)
+    out["config2_48_stars"] = S.config2(48, 48)
+    sc = S.config1(24, 16)
+    sc.sqrt_spp = 2
+    sc.hs_enabled, sc.hs_omega, sc.observer_time = 1, 1 / (8 * math.sqrt(8) + 0.9), 40.0
+    out["spp2_hotspot"] = sc
+    sc = S.default_scene()
+    sc.M_kg, sc.spin, sc.cam_r, sc.disc_r_out, sc.eddington = 1e9 * M_SUN, 0.998, 80.0, 20.0, 0.5
+    sc.cam_theta, sc.cam_fov = S.inclination(84), 0.9
+    sc.jet_enabled, sc.jet_z_top, sc.jet_gamma_max = 1, 130.0, 10.0
+    sc.width, sc.height, sc.sqrt_spp, sc.star_count = 32, 18, 1, 20000
+    oracle.ref().bhref_setup_jet(C.byref(sc))
+    sc.r_escape = S.escape_radius(sc)
+    out["quasar_jets"] = sc
+    sc = S.default_scene()
+    sc.M_kg, sc.spin, sc.disc_enabled, sc.ns_enabled = 1.4 * M_SUN, 0.0, 0, 1
+    sc.ns_R, sc.ns_caps, sc.ns_Omega = 6.9, 1, 0.002
+    sc.cam_r = 40 * sc.ns_R
+    sc.cam_fov = 5 * math.atan2(sc.ns_R / math.sqrt(1 - 2 / sc.ns_R), sc.cam_r)
+    sc.cam_theta = S.inclination(60)
+    sc.width, sc.height, sc.sqrt_spp, sc.star_count = 32, 32, 1, 20000
+    sc.r_escape = S.escape_radius(sc)
+    out["neutron_star_caps"] = sc
+    sc = S.default_scene()
+    sc.M_kg, sc.spin, sc.cam_r, sc.disc_r_out, sc.eddington = 4.297e6 * M_SUN, 0.9, 150.0, 26.0, 1e-6
+    sc.cam_theta, sc.cam_fov, sc.cam_yaw = S.inclination(82), 0.55, 0.2
+    sc.width, sc.height, sc.sqrt_spp, sc.star_count = 32, 18, 1, 20000
+    sc.sun_enabled, sc.sun_radius = 1, 30.0
+    sc.sun_centre[1] = 60.0
+    sc.r_escape = S.escape_radius(sc)
+    out["sun"] = sc
+    return out
+
+
+def main():
+    if not oracle.ref_available():
+        raise SystemExit("reference shim not built: make -C oracle (needs /root/reference)")
+    meta = {"scenes": {}, "note": "written by tests/golden/make_golden.py from oracle/_ref"}
+    arrays = {}
+    for name, sc in scenes().items():
+        xyz, st = oracle.ref_render(sc)
+        stride = 3 if sc.n_rays > 2000 else 1
+        ids = np.arange(0, sc.n_rays, stride)
+        r = oracle.ref_trace_list(sc, ids)
+        meta["scenes"][name] = {
+            "scene": scene_dict(sc),
+            "xyz_sha256": hashlib.sha256(np.ascontiguousarray(xyz).tobytes()).hexdigest(),
+            "stats": {k: getattr(st, k) for k in ("rays", "steps", "captured", "disc_hits", "sun_hits",
+                                                  "star_hits", "jet_rays", "escaped", "exhausted",
+                                                  "max_norm_error", "max_carter_drift")},
+        }
+        arrays[f"{name}/ids"] = ids
+        for k in ("cls", "y", "jet", "steps", "g", "ray_xyz"):
+            arrays[f"{name}/{k}"] = r[k]
+        arrays[f"{name}/xyz_row0"] = xyz[0]
+    # colour table samples (spec::blackbody_xyz)
+    T = np.concatenate([[10.0, 50.0, 51.3, 999.0, 5772.0, 1e4, 2.5e7, 3e9, 1e10],
+                        np.geomspace(60.0, 2e9, 200)])
+    bb = np.zeros((len(T), 3))
+    for i, t in enumerate(T):
+        oracle.ref().bhref_blackbody_xyz(float(t), bb[i].ctypes.data_as(C.c_void_p))
+    arrays["bb/T"], arrays["bb/xyz"] = T, bb
+    # star field radiance (StarField(20000, seed) + Milky Way)
+    rng = np.random.default_rng(3)
+    d = rng.normal(size=(300, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    sig = 4.0e-3
+    rad = np.zeros((300, 3))
+    for i in range(300):
+        di = np.ascontiguousarray(d[i])
+        oracle.ref().bhref_star_radiance(20000, 20240921, di.ctypes.data_as(C.c_void_p), sig,
+                                         rad[i].ctypes.data_as(C.c_void_p))
+    arrays["stars/dir"], arrays["stars/xyz"] = d, rad
+    meta["stars"] = {"count": 20000, "seed": 20240921, "sigma": sig}
+    # display transform
+    xyz = np.abs(rng.lognormal(0, 3, size=(4000, 3))) * 1e-3
+    xyz[::97] = 0.0
+    e = oracle.ref().bhref_auto_exposure(xyz.ctypes.data_as(C.c_void_p), len(xyz), 1.0)
+    arrays["display/xyz"] = xyz
+    meta["display"] = {"exposure": e}
+    for tm in range(5):
+        rgb = np.zeros((len(xyz), 3), np.uint8)
+        oracle.ref().bhref_develop(xyz.ctypes.data_as(C.c_void_p), len(xyz), e, tm, 16.0,
+                                   rgb.ctypes.data_as(C.c_void_p))
+        arrays[f"display/rgb{tm}"] = rgb
+    # TOV: Gamma = 2, K = 100 (G = c = Msun = 1) polytrope, K in SI for P = K rho^2
+    L = G * M_SUN / C_LIGHT ** 2
+    K_si = 100.0 * L ** 3 * C_LIGHT ** 2 / M_SUN
+    tov = []
+    for m in (1.2, 1.4, 1.6, 1.8):
+        R = C.c_double()
+        Mo = C.c_double()
+        rc = oracle.ref().bhref_polytrope_star(K_si, 2.0, m * M_SUN, C.byref(R), C.byref(Mo))
+        tov.append({"M_target": m, "ok": rc == 0, "R_m": R.value, "M_kg": Mo.value})
+    meta["tov"] = {"K_si": K_si, "Gamma": 2.0, "stars": tov}
+    # Jet::configure through the CLI's setup_jet
+    jets = []
+    for spin, edd, z_top in ((0.998, 0.5, 130.0), (0.94, 1e-5, 130.0), (0.7, 0.1, 120.0)):
+        sc = S.default_scene()
+        sc.M_kg, sc.spin, sc.eddington, sc.disc_r_out, sc.jet_z_top = 1e9 * M_SUN, spin, edd, 20.0, z_top
+        sc.jet_enabled = 1
+        oracle.ref().bhref_setup_jet(C.byref(sc))
+        jets.append({"spin": spin, "eddington": edd, "z_top": z_top, "j_nu0": sc.jet_j_nu0,
+                     "r_bound": sc.jet_r_bound, "Omega_F": sc.jet_Omega_F, "z_sat": sc.jet_z_sat})
+    meta["jets"] = jets
+    np.savez_compressed(os.path.join(HERE, "reference_outputs.npz"), **arrays)
+    with open(os.path.join(HERE, "reference_outputs.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print("wrote", len(arrays), "arrays")
+
+
+if __name__ == "__main__":
+    main()
