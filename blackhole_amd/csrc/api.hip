@@ -77,6 +77,10 @@ struct bhx_context {
     int n_tiles = 0;
     // initial ray states in queue order (kInitRows x n) and slot -> ray map
     DevBuf init, q2ray;
+    // jet / Sun parameters of the tracer's event branches (last uploaded copy)
+    DevBuf evp;
+    EventParams ev_host{};
+    bool ev_valid = false;
     // counters
     Counters* d_cnt = nullptr;
     DevBuf flux_partial, lum_hist, lum_npos;
@@ -232,7 +236,7 @@ int bhx_destroy(bhx_context* ctx) {
     if (ctx->d_bb) (void)hipFree(ctx->d_bb);
     if (ctx->d_cnt) (void)hipFree(ctx->d_cnt);
     for (DevBuf* b : {&ctx->cell_start, &ctx->star_rows, &ctx->tile_order, &ctx->tile_start,
-                      &ctx->init, &ctx->q2ray, &ctx->flux_partial,
+                      &ctx->init, &ctx->q2ray, &ctx->evp, &ctx->flux_partial,
                       &ctx->lum_hist, &ctx->lum_npos, &ctx->r_cls, &ctx->r_y, &ctx->r_jet, &ctx->r_xyz})
         b->release();
     for (auto& e : ctx->ev)
@@ -304,6 +308,19 @@ int bhx_trace_dev(bhx_context* ctx, const bhx_scene* s, const bhx_rays* rays, bh
     G.init = static_cast<double*>(ctx->init.p);
     G.q2ray = static_cast<int32_t*>(ctx->q2ray.p);
     TraceArgs P = host::trace_args(*s);
+    HIPCHK(ctx->evp.ensure(sizeof(EventParams)));
+    P.ev = static_cast<const EventParams*>(ctx->evp.p);
+    if (P.has_jet || P.has_sun) {
+        // synchronous and only on change: the buffer may still be read by a
+        // launch in flight, and a pageable source cannot be copied asynchronously
+        const EventParams evh = host::event_params(*s);
+        if (!ctx->ev_valid || std::memcmp(&evh, &ctx->ev_host, sizeof evh) != 0) {
+            HIPCHK(hipStreamSynchronize(st));
+            HIPCHK(hipMemcpy(ctx->evp.p, &evh, sizeof evh, hipMemcpyHostToDevice));
+            ctx->ev_host = evh;
+            ctx->ev_valid = true;
+        }
+    }
     P.refill = std::max(1, std::min(64, ctx->launch.refill));
     P.chunk = std::max(1, ctx->launch.chunk);
     P.init = G.init;

@@ -253,6 +253,21 @@ RaygenArgs raygen_args(const bhx_scene& s) {
     return P;
 }
 
+EventParams event_params(const bhx_scene& s) {
+    EventParams E{};
+    E.z_base = s.jet_z_base; E.z_top = s.jet_z_top; E.R_base = s.jet_R_base; E.k = s.jet_k;
+    E.gamma_max = s.jet_gamma_max; E.z_sat = s.jet_z_sat; E.alpha = s.jet_alpha;
+    E.u_peak = s.jet_u_peak; E.u_wid = s.jet_u_wid; E.spine = s.jet_spine;
+    E.spine_wid = s.jet_spine_wid; E.zeta = s.jet_zeta; E.Omega_F = s.jet_Omega_F;
+    E.j_nu0 = s.jet_j_nu0; E.r_bound = s.jet_r_bound;
+    // Jet::gamma_at(z_base) (jet.h:159-162)
+    const double t = std::min(1.0, std::pow(std::max(s.jet_z_base, 1e-6) / s.jet_z_sat, s.jet_k));
+    E.gamma_zb = 1.0 + (s.jet_gamma_max - 1.0) * t;
+    for (int i = 0; i < 3; ++i) E.sun_c[i] = s.sun_centre[i];
+    E.sun_R = s.sun_radius;
+    return E;
+}
+
 TraceArgs trace_args(const bhx_scene& s) {
     TraceArgs P{};
     const double a = s.spin;
@@ -274,6 +289,8 @@ TraceArgs trace_args(const bhx_scene& s) {
     }
     P.has_ns = s.ns_enabled != 0;
     P.ns_R = s.ns_R;
+    P.has_jet = s.jet_enabled != 0;
+    P.has_sun = s.sun_enabled != 0;
     const int Sspp = std::max(1, s.sqrt_spp);
     P.n_rays = s.width * s.height * Sspp * Sspp;
     return P;
@@ -335,6 +352,11 @@ ShadeArgs shade_args(const bhx_scene& s) {
         P.mw_cz = c[2] * inv;
     }
     if (s.jet_enabled) powerlaw_xyz(s.jet_alpha, P.jet_rgb);
+    P.has_sun = s.sun_enabled;
+    for (int i = 0; i < 3; ++i) P.sun_c[i] = s.sun_centre[i];
+    P.sun_R = s.sun_radius;
+    P.sun_T_eff = s.sun_T_eff;
+    P.sun_self_z = kG * 1.98847e30 / (6.957e8 * kC * kC);   // G M_sun / (R_sun c^2), scene.h:239-240
     return P;
 }
 

@@ -35,6 +35,7 @@ void bin_catalogue(const Catalogue& cat, std::vector<uint32_t>& cell_start,
 
 // Scene -> kernel arguments.  Pointers and queue fields are left for the caller.
 RaygenArgs raygen_args(const bhx_scene& s);
+EventParams event_params(const bhx_scene& s);
 TraceArgs trace_args(const bhx_scene& s);
 ShadeArgs shade_args(const bhx_scene& s);
 

@@ -5,6 +5,14 @@
 
 namespace bhx {
 
+// Jet (after Jet::configure, src/jet.h:96-145) and Sun (src/scene.h:232-235)
+// parameters of the tracer's event branches, read from device memory.
+struct EventParams {
+    double z_base, z_top, R_base, k, gamma_max, z_sat, alpha, u_peak, u_wid, spine, spine_wid;
+    double zeta, Omega_F, j_nu0, r_bound, gamma_zb;   // gamma_zb = Jet::gamma_at(z_base)
+    double sun_c[3], sun_R;
+};
+
 // Ray generation: camera, jitter and the initial photon of every sample
 // (src/render.cpp:437-452, src/scene.h:356-384, src/geodesic.h:215-231),
 // written in work-queue order so the tracer's refills are coalesced loads.
@@ -39,7 +47,8 @@ struct TraceArgs {
     double pw_init;                 // pow(1e-4, 0.08): controller memory of a fresh ray
     // events (src/render.cpp:158-199)
     double r_in, r_out, ns_R;
-    int32_t has_disc, has_ns, max_steps, refill, chunk;
+    int32_t has_disc, has_ns, has_jet, has_sun, max_steps, refill, chunk;
+    const EventParams* ev;          // jet and Sun parameters (device memory)
     int32_t n_rays;
     const double* init;             // kInitRows x n_rays, queue order
     const int32_t* q2ray;
@@ -79,6 +88,9 @@ struct ShadeArgs {
     const uint32_t* cell_start;               // n_cells + 1 offsets into star rows
     const double* star;                       // 6 doubles per star: x y z irr cx cz
     double mw_L, mw_cx, mw_cz;                // Milky Way amplitude and colour ratios
+    // Sun (render.cpp:301-325, scene.h:239-276)
+    int32_t has_sun;
+    double sun_c[3], sun_R, sun_T_eff, sun_self_z;
     // jet colour (spec::powerlaw_xyz of the jet's alpha)
     double jet_rgb[3];
     // spectrum table: 3 x 4096 natural logs (spectrum.cpp:97-114)

@@ -197,6 +197,43 @@ __global__ void __launch_bounds__(256) k_shade(const ShadeArgs P) {
                     bb_lookup(P.bb, g * T, col);
                     col[0] *= limb; col[1] *= limb; col[2] *= limb;
                 }
+            } else if (cls == BHX_SUN && P.has_sun) {
+                // the Sun, at rest in Boyer-Lindquist coordinates (render.cpp:301-325)
+                const double r = y[1];
+                double sn, cs;
+                sincos(y[2], &sn, &cs);
+                const double S = r * r + P.a2 * cs * cs;
+                const double gtt = -(1.0 - 2.0 * r / S);
+                if (gtt < 0.0) {
+                    const double ku = y[4] * (1.0 / sqrt(-gtt));
+                    g = (ku > 1e-12) ? 1.0 / ku : 0.0;
+                    if (g > 0.0) {
+                        double sp, cp;
+                        sincos(y[3], &sp, &cp);
+                        const double R = sqrt(r * r + P.a2);
+                        const double px = R * sn * cp, py = R * sn * sp, pz = r * cs;
+                        Rhs f;
+                        kerr_rhs(ray_k(P.a, P.a2, y[4], y[7]), y[1], y[2], y[5], y[6], f);
+                        const double dr = f.d[1], dth = f.d[2], dph = f.d[3];
+                        const double dR = (R > 0.0) ? r / R * dr : dr;
+                        double v0 = dR * sn * cp + R * cs * dth * cp - R * sn * sp * dph;
+                        double v1 = dR * sn * sp + R * cs * dth * sp + R * sn * cp * dph;
+                        double v2 = dr * cs - r * sn * dth;
+                        const double vn = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+                        if (vn > 0.0) { v0 /= vn; v1 /= vn; v2 /= vn; }
+                        // Eddington limb darkening (scene.h:266-276)
+                        double limb = 1.0;
+                        double n0 = px - P.sun_c[0], n1 = py - P.sun_c[1], n2 = pz - P.sun_c[2];
+                        const double len = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+                        if (len > 0.0) {
+                            n0 /= len; n1 /= len; n2 /= len;
+                            const double mu = dclamp(-(n0 * v0 + n1 * v1 + n2 * v2), 0.0, 1.0);
+                            limb = (2.0 + 3.0 * mu) / 5.0;
+                        }
+                        bb_lookup(P.bb, g * P.sun_T_eff / (1.0 + P.sun_self_z), col);
+                        col[0] *= limb; col[1] *= limb; col[2] *= limb;
+                    }
+                }
             } else if (cls == BHX_SKY) {
                 // escape direction (render.cpp:15-26, 358-368)
                 Rhs f;

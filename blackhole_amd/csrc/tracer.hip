@@ -21,6 +21,7 @@
 //             by ballot + prefix rank (mbcnt) - consecutive slots, so the
 //             refill loads are coalesced; one loop iteration = one DP5 attempt
 //             on every live lane.
+#include "events_dev.h"
 #include "kerr_dev.h"
 #include "params.h"
 #include "../../include/bhx.h"
@@ -52,6 +53,12 @@ constexpr double kAlpha = 0.7 / 5.0, kBeta = 0.4 / 5.0;
 constexpr double kLog1em4 = -9.210340371976182;  // log(1e-4): err_prev floor
 
 constexpr int kTile = 8;
+
+// Four-point Gauss-Legendre nodes and weights on [0, 1] (render.cpp:47-50).
+__constant__ const double kGLx[4] = {0.06943184420297371, 0.33000947820757187, 0.66999052179242813,
+                                     0.93056815579702629};
+__constant__ const double kGLw[4] = {0.17392742256872693, 0.32607257743127307, 0.32607257743127307,
+                                     0.17392742256872693};
 
 __device__ __forceinline__ double dense_comp(double c1, double c2, double c3, double c4, double c5,
                                              double s) {
@@ -223,7 +230,7 @@ k_trace(const TraceArgs P) {
     double y[6], k1[6];                 // t, r, th, ph, p_r, p_th
     double pt = 0.0, pph = 0.0;         // conserved p_t, p_phi
     double h = 0.0, pw = 0.0, c_prev = 0.0, q0raw = 0.0, iq0 = 0.0;
-    double mx_norm = 0.0, mx_carter = 0.0;
+    double mx_norm = 0.0, mx_carter = 0.0, jet_acc = 0.0;
     int nsteps = 0, natt = 0, tries = 0;
 #pragma unroll
     for (int c = 0; c < 6; ++c) { y[c] = 0.0; k1[c] = 0.0; }
@@ -267,7 +274,7 @@ k_trace(const TraceArgs P) {
                     h = P.h0;
                     pw = P.pw_init;
                     nsteps = natt = tries = 0;
-                    mx_norm = mx_carter = 0.0;
+                    mx_norm = mx_carter = jet_acc = 0.0;
                     live = P.max_steps > 0;
                     if (!live) {  // the step loop never runs: Exhausted at y0
                         const int ray = P.q2ray[q];
@@ -376,8 +383,32 @@ k_trace(const TraceArgs P) {
                 mx_carter = dmax(mx_carter, fabs(Q - q0raw) * iq0);
             }
             const bool ns_hit = P.has_ns && yn[1] <= P.ns_R;
-            const bool disc_cross = P.has_disc && c_prev * g7.cs < 0.0;
-            if (ns_hit || (disc_cross && yn[1] > P.r_stop)) {
+            const bool horizon = !ns_hit && yn[1] <= P.r_stop;
+            const bool disc_cross = !ns_hit && !horizon && P.has_disc && c_prev * g7.cs < 0.0;
+            // jet cull and Sun candidate need the pseudo-Cartesian end points of the step
+            bool near_jet = false, sun_cand = false;
+            double p0[3], p1[3];
+            if (P.has_jet || P.has_sun) {
+                to_cart(P.a2, y[1], y[2], y[3], p0);
+                to_cart(P.a2, yn[1], yn[2], yn[3], p1);
+                if (P.has_jet) near_jet = seg_dist(p0, p1) < P.ev->r_bound;          // render.cpp:151-152
+                if (P.has_sun && !ns_hit && !horizon) {                             // render.cpp:203-209
+                    const double* C = P.ev->sun_c;
+                    const double e0 = p1[0] - p0[0], e1 = p1[1] - p0[1], e2 = p1[2] - p0[2];
+                    const double seg_len = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+                    const double f0 = p0[0] - C[0], f1 = p0[1] - C[1], f2 = p0[2] - C[2];
+                    sun_cand = sqrt(f0 * f0 + f1 * f1 + f2 * f2) <= seg_len + P.ev->sun_R * 1.5;
+                }
+            }
+            double s_jet = -1.0;   // fraction of the step the jet integral covers
+            if (horizon) {
+                // ---- horizon (render.cpp:172-177): after the NS test, before the disc
+#pragma unroll
+                for (int c = 0; c < 6; ++c) yo[c] = yn[c];
+                term = BHX_HORIZON;
+                s_jet = 1.0;
+            }
+            if (ns_hit || disc_cross || sun_cand || near_jet) {
                 // dense output of the step (geodesic.h:177-185)
                 double c1[6], c2[6], c3[6], c4[6];
 #pragma unroll
@@ -393,7 +424,9 @@ k_trace(const TraceArgs P) {
 #pragma unroll
                     for (int c = 0; c < 6; ++c) yo[c] = dense_comp(c1[c], c2[c], c3[c], c4[c], c5[c], se);
                     term = BHX_STAR;
-                } else {
+                    s_jet = se;
+                }
+                if (disc_cross) {
                     // ---- equatorial crossing (render.cpp:180-199) ---------------
                     const double se = root_theta(c1[2], c2[2], c3[2], c4[2], c5[2], c_prev, y[2], yn[2]);
                     const double rc = dense_comp(c1[1], c2[1], c3[1], c4[1], c5[1], se);
@@ -401,16 +434,70 @@ k_trace(const TraceArgs P) {
 #pragma unroll
                         for (int c = 0; c < 6; ++c) yo[c] = dense_comp(c1[c], c2[c], c3[c], c4[c], c5[c], se);
                         term = BHX_DISC;
+                        s_jet = se;
                     }
                 }
-            }
-            if (term < 0 && yn[1] <= P.r_stop) {
-                // ---- horizon (render.cpp:172-177): after the NS test, before the disc
+                if (term < 0 && sun_cand) {
+                    // ---- the Sun: 12 probes then 50 bisections (render.cpp:210-240)
+                    const double* C = P.ev->sun_c;
+                    const double R2 = P.ev->sun_R * P.ev->sun_R;
+                    auto fsun = [&](double sv) {
+                        double q[3];
+                        to_cart(P.a2, dense_comp(c1[1], c2[1], c3[1], c4[1], c5[1], sv),
+                                dense_comp(c1[2], c2[2], c3[2], c4[2], c5[2], sv),
+                                dense_comp(c1[3], c2[3], c3[3], c4[3], c5[3], sv), q);
+                        const double dx = q[0] - C[0], dy = q[1] - C[1], dz = q[2] - C[2];
+                        return dx * dx + dy * dy + dz * dz - R2;
+                    };
+                    if (fsun(0.0) <= 0.0) {  // camera already inside: a hit at the step start
 #pragma unroll
-                for (int c = 0; c < 6; ++c) yo[c] = yn[c];
-                term = BHX_HORIZON;
-            } else if (term < 0 && yn[1] > P.r_escape) {
-                // ---- escape (render.cpp:247-251) --------------------------------
+                        for (int c = 0; c < 6; ++c) yo[c] = y[c];
+                        term = BHX_SUN;
+                    } else {
+                        double prev = 0.0;
+                        for (int kp = 1; kp <= 12 && term < 0; ++kp) {
+                            const double sv = (double)kp / 12;
+                            if (fsun(sv) <= 0.0) {
+                                double lo = prev, hi = sv;
+                                for (int it = 0; it < 50; ++it) {
+                                    const double mid = 0.5 * (lo + hi);
+                                    if (fsun(mid) <= 0.0) hi = mid; else lo = mid;
+                                }
+                                const double se = 0.5 * (lo + hi);
+#pragma unroll
+                                for (int c = 0; c < 6; ++c) yo[c] = dense_comp(c1[c], c2[c], c3[c], c4[c], c5[c], se);
+                                term = BHX_SUN;
+                                s_jet = se;
+                            }
+                            prev = sv;
+                        }
+                    }
+                }
+                if (term < 0) s_jet = 1.0;   // nothing stopped the ray: the whole step counts
+                if (near_jet && s_jet > 0.0) {
+                    // ---- jet emission: 4-point Gauss-Legendre (render.cpp:107-122)
+                    const EventParams& J = *P.ev;
+                    double sum = 0.0;
+#pragma unroll
+                    for (int qd = 0; qd < 4; ++qd) {
+                        const double sv = kGLx[qd] * s_jet;
+                        const double rs = dense_comp(c1[1], c2[1], c3[1], c4[1], c5[1], sv);
+                        const double ths = dense_comp(c1[2], c2[2], c3[2], c4[2], c5[2], sv);
+                        double sn, cs;
+                        sincos_near(ths, &sn, &cs);
+                        const double sh = jet_shape(J, rs * cs, sqrt(rs * rs + P.a2) * sn);
+                        if (sh <= 0.0) continue;
+                        const double b = jet_boost(J, P.a, P.a2, rs, ths, pt,
+                                                   dense_comp(c1[4], c2[4], c3[4], c4[4], c5[4], sv),
+                                                   dense_comp(c1[5], c2[5], c3[5], c4[5], c5[5], sv), pph);
+                        if (b <= 0.0) continue;
+                        sum += kGLw[qd] * sh * b;
+                    }
+                    jet_acc += J.j_nu0 * sum * h * s_jet;
+                }
+            }
+            if (term < 0 && yn[1] > P.r_escape) {
+                // ---- escape (render.cpp:247-251), after the jet sample ----------
 #pragma unroll
                 for (int c = 0; c < 6; ++c) yo[c] = yn[c];
                 term = BHX_SKY;
@@ -446,7 +533,7 @@ k_trace(const TraceArgs P) {
                 P.y[3 * M + ray] = yo[3]; P.y[4 * M + ray] = pt; P.y[5 * M + ray] = yo[4];
                 P.y[6 * M + ray] = yo[5]; P.y[7 * M + ray] = pph;
             }
-            if (P.jet) P.jet[ray] = 0.0;
+            if (P.jet) P.jet[ray] = jet_acc;
             if (P.steps) P.steps[ray] = nsteps;
             if (P.attempts) P.attempts[ray] = natt;
             if (P.norm_err) P.norm_err[ray] = mx_norm;
