@@ -18,7 +18,7 @@ HIP_SOURCES = ["tracer.hip", "shade.hip", "api.hip"]
 # literals, so machine LICM would pin every one of them in two registers for
 # the whole loop and spill.  Without it they are rematerialised where used.
 EXTRA = {"tracer.hip": ["-mllvm", "-disable-machine-licm"]}
-CXX_SOURCES = ["host_setup.cpp"]
+CXX_SOURCES = ["host_setup.cpp", "host_physics.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BHX_ARCH", "gfx950")

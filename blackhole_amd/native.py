@@ -16,7 +16,8 @@ P = C.c_void_p
 SYMBOLS = (
     "bhx_create", "bhx_destroy", "bhx_last_error", "bhx_abi_version", "bhx_set_catalogue",
     "bhx_render", "bhx_trace_dev", "bhx_shade_dev", "bhx_auto_exposure_dev", "bhx_develop_dev",
-    "bhx_blackbody_xyz_dev", "bhx_set_launch", "bhx_get_launch",
+    "bhx_blackbody_xyz_dev", "bhx_set_launch", "bhx_get_launch", "bhx_setup_jet",
+    "bhx_polytrope_star", "bhx_setup_neutron_star",
 )
 
 
@@ -48,6 +49,10 @@ def lib():
     L.bhx_blackbody_xyz_dev.argtypes = [P, P, C.c_int64, P, P]
     L.bhx_set_launch.argtypes = [P, C.POINTER(Launch)]
     L.bhx_get_launch.argtypes = [P, C.POINTER(Launch)]
+    L.bhx_setup_jet.argtypes = [C.POINTER(Scene)]
+    L.bhx_polytrope_star.argtypes = [C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]
+    L.bhx_setup_neutron_star.argtypes = [C.POINTER(Scene), C.c_double, C.c_double]
     _lib = L
     return L
 

@@ -8,8 +8,9 @@ way a user of the reference describes it:
   * the inclination clamp to [0.5, 179.5] degrees (src/main.cpp:810);
   * the escape-radius rule r_escape = max(3 r_cam + 100, ...) (src/main.cpp:882-890).
 The numerical host set-up (Jet::configure, TOV) is native code in the C ABI
-library (blackhole_amd/csrc/host_setup.cpp).
+library (blackhole_amd/csrc/host_physics.cpp); it makes no GPU call.
 """
+import ctypes as C
 import math
 
 from .abi import Scene
@@ -94,3 +95,36 @@ def config2(width=1024, height=1024, stars=True):
     0.3 Eddington, 250k-star sky (+-55M image plane, 10 Msun)."""
     return plain_scene(spin=0.998, incl_deg=60.0, r_cam=1000.0, width=width, height=height,
                        fov=2.0 * math.atan(0.055), disc_r_out=50.0, eddington=0.3, stars=stars)
+
+
+def setup_jet(s):
+    """Normalise the scene's jets in place (src/main.cpp:399-423, src/jet.h:261-311);
+    clears jet_enabled where the reference's CLI switches the jets off."""
+    from . import native
+    native.check(None, native.lib().bhx_setup_jet(C.byref(s)), "bhx_setup_jet")
+    return s
+
+
+def polytrope_star(K_si, gamma, M_kg):
+    """(R_m, M_kg) of the stable star of mass M_kg for P = K rho^gamma
+    (src/tov.h:205-218); BhxError when the equation of state cannot hold it."""
+    from . import native
+    R, M = C.c_double(), C.c_double()
+    rc = native.lib().bhx_polytrope_star(float(K_si), float(gamma), float(M_kg), C.byref(R), C.byref(M))
+    if rc != 0:
+        raise native.BhxError(f"polytrope K={K_si} Gamma={gamma} cannot support M={M_kg} kg")
+    return R.value, M.value
+
+
+def neutron_star_scene(s, R_m, M_kg, *, frame=True):
+    """Turn s into a neutron-star scene for a solved star (src/main.cpp:503-520):
+    Schwarzschild exterior, no disc, no jet; with frame=True the camera is put
+    at 40 R and the field of view fitted to the lensed radius R/sqrt(1-2/R)."""
+    from . import native
+    native.check(None, native.lib().bhx_setup_neutron_star(C.byref(s), float(R_m), float(M_kg)),
+                 "bhx_setup_neutron_star")
+    if frame:
+        s.cam_r = 40.0 * s.ns_R
+        s.cam_fov = 5.0 * math.atan2(s.ns_R / math.sqrt(1.0 - 2.0 / s.ns_R), s.cam_r)
+    s.r_escape = escape_radius(s)
+    return s

@@ -204,6 +204,28 @@ typedef struct bhx_launch {
 int bhx_set_launch(bhx_context *ctx, const bhx_launch *launch);
 int bhx_get_launch(const bhx_context *ctx, bhx_launch *launch);
 
+/* ---- scene preparation (host only, no context, no GPU) ------------------ */
+
+/* Normalise the jets of a scene from its spin and accretion rate: the CLI's
+ * setup_jet (src/main.cpp:399-423) with Jet::configure (src/jet.h:261-311).
+ * Reads M_kg, spin, eddington, disc_sense, jet_z_top, jet_z_sat and the jet
+ * shape fields; writes jet_z_sat (capped at 0.7 z_top), jet_a, jet_Omega_F,
+ * jet_j_nu0, jet_r_bound.  Clears jet_enabled when the scene has no disc, is a
+ * neutron-star scene, or the Blandford-Znajek power is zero (a = 0). */
+int bhx_setup_jet(bhx_scene *scene);
+
+/* Stable-branch star of gravitational mass M_kg for the polytrope
+ * P = K_si rho^Gamma (SI units): ns::star_of_mass (src/tov.h:205-218) over
+ * ns::solve_tov (src/tov.h:65-177).  Returns BHX_E_INVALID when the equation
+ * of state cannot hold that mass. */
+int bhx_polytrope_star(double K_si, double Gamma, double M_kg, double *R_m, double *M_out);
+
+/* Make a scene a neutron-star scene for a solved star (R_m metres, M_kg kg):
+ * the mass and spin rules of the CLI's setup_neutron_star (src/main.cpp:
+ * 503-512) and NeutronStar::set_from / set_spin (src/neutronstar.h:69-79);
+ * ns_spin_hz must already be set.  Camera framing stays with the caller. */
+int bhx_setup_neutron_star(bhx_scene *scene, double R_m, double M_kg);
+
 #ifdef __cplusplus
 }
 #endif
