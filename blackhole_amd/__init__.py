@@ -8,7 +8,7 @@ library libbhx.so through the C ABI of include/bhx.h.
 
     from blackhole_amd import Tracer, scene
     with Tracer() as tr:
-        xyz, stats = tr.render(scene.config1())
+        xyz, stats = tr.render(scene.plain_frame())
 """
 from . import abi, scene
 from .abi import CLASS_NAMES, Scene, Stats

@@ -164,8 +164,8 @@ Star star_of_mass(const Tov& eos, double M_target, double rho_lo = 2e17, double 
 
 extern "C" {
 
-int bhx_setup_jet(bhx_scene* s) {
-    if (!s) return BHX_E_INVALID;
+int bhx_setup_jet(bhx_scene* s, double phi_flux, double eps_rad) {
+    if (!s || !(phi_flux >= 0.0) || !(eps_rad >= 0.0)) return BHX_E_INVALID;
     if (!s->jet_enabled) return BHX_OK;
     if (s->ns_enabled || !s->disc_enabled) {  // no horizon / no accretion flow: no jet
         s->jet_enabled = 0;
@@ -188,7 +188,7 @@ int bhx_setup_jet(bhx_scene* s) {
     const double OmH = horizon_omega(spin);
     const double x = OmH * OmH;
     const double f = 1.0 + 1.38 * x - 9.2 * x * x;
-    const double phi = 50.0, kappa = 0.053, eps_rad = 0.02;          // jet.h:140-141, 82
+    const double phi = phi_flux, kappa = 0.053;                      // jet.h:82
     const double P_jet = (kappa / (4.0 * kPi)) * phi * phi * x * std::max(0.0, f) * mdot_c2;
     const double L_rad = eps_rad * P_jet;
     const double r_bound = std::sqrt(j.z_top * j.z_top + j.radius(j.z_top) * j.radius(j.z_top)) * 1.05;

@@ -49,7 +49,7 @@ def lib():
     L.bhx_blackbody_xyz_dev.argtypes = [P, P, C.c_int64, P, P]
     L.bhx_set_launch.argtypes = [P, C.POINTER(Launch)]
     L.bhx_get_launch.argtypes = [P, C.POINTER(Launch)]
-    L.bhx_setup_jet.argtypes = [C.POINTER(Scene)]
+    L.bhx_setup_jet.argtypes = [C.POINTER(Scene), C.c_double, C.c_double]
     L.bhx_polytrope_star.argtypes = [C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]
     L.bhx_setup_neutron_star.argtypes = [C.POINTER(Scene), C.c_double, C.c_double]

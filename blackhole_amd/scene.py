@@ -82,7 +82,7 @@ def plain_scene(*, spin, incl_deg, r_cam, width, height, fov, disc_r_out, edding
     return s
 
 
-def config1(width=256, height=256):
+def plain_frame(width=256, height=256):
     """Plain FP64 render: a=0.9, 10 Msun, r=500, 80 deg, +-20M image plane,
     NT disc ISCO-20M at 0.1 Eddington, no stars, rtol 1e-10 / atol 1e-12."""
     return plain_scene(spin=0.9, incl_deg=80.0, r_cam=500.0, width=width, height=height,
@@ -90,18 +90,20 @@ def config1(width=256, height=256):
                        rtol=1e-10, atol=1e-12)
 
 
-def config2(width=1024, height=1024, stars=True):
+def starry_frame(width=1024, height=1024, stars=True):
     """Plain FP64 render at 1024^2: a=0.998, 60 deg, r=1000, NT disc ISCO-50M at
     0.3 Eddington, 250k-star sky (+-55M image plane, 10 Msun)."""
     return plain_scene(spin=0.998, incl_deg=60.0, r_cam=1000.0, width=width, height=height,
                        fov=2.0 * math.atan(0.055), disc_r_out=50.0, eddington=0.3, stars=stars)
 
 
-def setup_jet(s):
+def setup_jet(s, phi_flux=50.0, eps_rad=0.02):
     """Normalise the scene's jets in place (src/main.cpp:399-423, src/jet.h:261-311);
-    clears jet_enabled where the reference's CLI switches the jets off."""
+    clears jet_enabled where the reference's CLI switches the jets off.  phi_flux
+    and eps_rad are the Jet defaults (src/jet.h:140-141)."""
     from . import native
-    native.check(None, native.lib().bhx_setup_jet(C.byref(s)), "bhx_setup_jet")
+    native.check(None, native.lib().bhx_setup_jet(C.byref(s), float(phi_flux), float(eps_rad)),
+                 "bhx_setup_jet")
     return s
 
 

@@ -9,7 +9,7 @@
  * have internal linkage, so the only seam a maintainer can replace is the
  * thread pool inside render(): the entry points below are what such a
  * replacement calls.  The C++ wrapper that keeps the reference's own
- * signature lives in blackhole_amd/csrc/host/render.h.
+ * signature is include/bhx_render.hpp.
  *
  * Conventions
  *   - Plain C: no C++ or torch types, fixed-size integer types, FP64 data.
@@ -208,11 +208,14 @@ int bhx_get_launch(const bhx_context *ctx, bhx_launch *launch);
 
 /* Normalise the jets of a scene from its spin and accretion rate: the CLI's
  * setup_jet (src/main.cpp:399-423) with Jet::configure (src/jet.h:261-311).
+ * phi_flux is the dimensionless horizon flux (Jet::phi_flux, default 50) and
+ * eps_rad the radiated fraction of the jet power (Jet::eps_rad, default 0.02),
+ * src/jet.h:140-141, the CLI's --jet-flux and --jet-efficiency.
  * Reads M_kg, spin, eddington, disc_sense, jet_z_top, jet_z_sat and the jet
  * shape fields; writes jet_z_sat (capped at 0.7 z_top), jet_a, jet_Omega_F,
  * jet_j_nu0, jet_r_bound.  Clears jet_enabled when the scene has no disc, is a
  * neutron-star scene, or the Blandford-Znajek power is zero (a = 0). */
-int bhx_setup_jet(bhx_scene *scene);
+int bhx_setup_jet(bhx_scene *scene, double phi_flux, double eps_rad);
 
 /* Stable-branch star of gravitational mass M_kg for the polytrope
  * P = K_si rho^Gamma (SI units): ns::star_of_mass (src/tov.h:205-218) over

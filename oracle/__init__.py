@@ -43,9 +43,11 @@ class OracleRays(C.Structure):
                 ("norm_err", P), ("carter", P), ("g", P), ("dir", P), ("ray_xyz", P)]
 
 
-def build():
-    """Compile lib/liboracle.so (and _ref/libbhref.so when the reference is present)."""
-    subprocess.run(["make", "-s", "-C", HERE], check=True)
+def build(cli=False):
+    """Compile lib/liboracle.so (and _ref/libbhref.so when the reference is present);
+    with cli=True also the reference's program, as shipped and with the GPU
+    render() linked in (needs blackhole_amd/lib/libbhx.so built first)."""
+    subprocess.run(["make", "-s", "-j8", "-C", HERE, "all"] + (["cli"] if cli else []), check=True)
 
 
 def lib():

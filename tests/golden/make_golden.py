@@ -40,9 +40,9 @@ def scene_dict(sc):
 
 def scenes():
     out = {}
-    out["config1_64"] = S.config1(64, 64)
-    out["config2_48_stars"] = S.config2(48, 48)
-    sc = S.config1(24, 16)
+    out["plain_64"] = S.plain_frame(64, 64)
+    out["starry_48"] = S.starry_frame(48, 48)
+    sc = S.plain_frame(24, 16)
     sc.sqrt_spp = 2
     sc.hs_enabled, sc.hs_omega, sc.observer_time = 1, 1 / (8 * math.sqrt(8) + 0.9), 40.0
     out["spp2_hotspot"] = sc

@@ -1,5 +1,5 @@
 """HIP tracer on the golden scenes (jets, Sun, neutron star with caps, spp 2 +
-hot spot, config 1 and a starry config 2), against the oracle ray by ray and
+hot spot, a 64x64 plain frame and a 48x48 starry frame), against the oracle ray by ray and
 against the reference's own stored outputs.  Needs an MI355X."""
 import json
 import os

@@ -1,7 +1,7 @@
 """HIP tracer vs the CPU oracle on identical scenes (needs an MI355X).
 
-Config 1 (256x256, a=0.9, rtol 1e-10) is checked ray by ray over the whole
-frame; config 2 (1024x1024, a=0.998, 250k stars) over a strided sample of
+The plain 256x256 frame (a=0.9, rtol 1e-10) is checked ray by ray over the whole
+frame; the starry 1024x1024 frame (a=0.998, 250k stars) over a strided sample of
 rays plus whole-frame properties.  The bar is the one of tests/parity.py.
 """
 import math
@@ -27,15 +27,15 @@ def _report(name, rep):
     print(f"\n[{name}] " + " ".join(f"{k}={v}" for k, v in rep.items()), flush=True)
 
 
-def test_config1_full_frame(tracer):
-    sc = S.config1()
-    print("[config1] 256x256 a=0.9 rtol 1e-10", flush=True)
+def test_plain_frame_256_full(tracer):
+    sc = S.plain_frame()
+    print("[plain 256] 256x256 a=0.9 rtol 1e-10", flush=True)
     rays, sh, st = _run(tracer, sc)
     g = gpu_rays_numpy(rays, sh)
     ids = np.arange(sc.n_rays)
     c = oracle.trace_list(sc, ids)
     rep = compare(sc, ids, g, c)
-    _report("config1", rep)
+    _report("plain256", rep)
     assert rep["fail"] == 0, rep
     assert rep["exempt"] <= 0.002 * rep["n"], rep
     # frame-level: classes, sRGB within 1 LSB with one exposure
@@ -60,15 +60,15 @@ def test_config1_full_frame(tracer):
     assert np.mean(g["steps"][same] == c["steps"][same]) > 0.99
 
 
-def test_config2_sampled(tracer):
-    sc = S.config2()
-    print("[config2] 1024x1024 a=0.998 250k stars", flush=True)
+def test_starry_frame_1024_sampled(tracer):
+    sc = S.starry_frame()
+    print("[starry 1024] 1024x1024 a=0.998 250k stars", flush=True)
     rays, sh, st = _run(tracer, sc)
     ids = np.arange(0, sc.n_rays, 37)
     g = {k: v[..., ids] for k, v in gpu_rays_numpy(rays, sh).items()}
     c = oracle.trace_list(sc, ids)
     rep = compare(sc, ids, g, c)
-    _report("config2", rep)
+    _report("starry1024", rep)
     assert rep["fail"] == 0, rep
     assert rep["exempt"] <= 0.002 * rep["n"], rep
     # colour of every sampled ray: within 1e-6 relative (a star's Gaussian PSF
@@ -83,7 +83,7 @@ def test_config2_sampled(tracer):
 @pytest.mark.parametrize("kind", ["spp2", "neutron_star", "hotspot"])
 def test_feature_scenes(tracer, kind):
     if kind == "spp2":
-        sc = S.config1(48, 32)
+        sc = S.plain_frame(48, 32)
         sc.sqrt_spp = 2
     elif kind == "neutron_star":
         sc = S.default_scene()
@@ -97,7 +97,7 @@ def test_feature_scenes(tracer, kind):
         sc.width, sc.height, sc.sqrt_spp, sc.star_count = 48, 48, 1, 20000
         sc.r_escape = S.escape_radius(sc)
     else:
-        sc = S.config1(64, 48)
+        sc = S.plain_frame(64, 48)
         sc.hs_enabled, sc.hs_r, sc.hs_omega, sc.observer_time = 1, 8.0, 1 / (8 * math.sqrt(8) + 0.9), 40.0
         sc.stars_enabled, sc.star_count = 1, 30000
     print(f"[{kind}] {sc.width}x{sc.height} spp {sc.sqrt_spp ** 2}", flush=True)
@@ -114,7 +114,7 @@ def test_feature_scenes(tracer, kind):
 
 
 def test_determinism(tracer):
-    sc = S.config1(128, 128)
+    sc = S.plain_frame(128, 128)
     r1, s1, _ = _run(tracer, sc)
     a = {k: v.clone() for k, v in r1.items()}
     x1 = s1["xyz"].clone()
@@ -125,7 +125,7 @@ def test_determinism(tracer):
 
 
 def test_render_dropin_matches_stages(tracer):
-    sc = S.config1(96, 64)
+    sc = S.plain_frame(96, 64)
     xyz, st = tracer.render(sc)
     rays, sh, st2 = _run(tracer, sc)
     assert np.array_equal(xyz, sh["xyz"].cpu().numpy())

@@ -36,16 +36,16 @@ def check_scene(sc, stride=3):
     assert np.array_equal(bits(a["g"][disc]), bits(b["g"][disc]))
 
 
-def test_config1_small():
-    check_scene(S.config1(40, 40))
+def test_plain_frame_small():
+    check_scene(S.plain_frame(40, 40))
 
 
-def test_config2_small_with_reference_stars():
-    check_scene(S.config2(40, 40))
+def test_starry_frame_small_with_reference_stars():
+    check_scene(S.starry_frame(40, 40))
 
 
 def test_spp_and_hotspot():
-    sc = S.config1(24, 16)
+    sc = S.plain_frame(24, 16)
     sc.sqrt_spp = 2
     sc.hs_enabled, sc.hs_omega, sc.observer_time = 1, 1 / (8 * math.sqrt(8) + 0.9), 40.0
     check_scene(sc, 1)
