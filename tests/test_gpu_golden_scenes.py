@@ -45,3 +45,32 @@ def test_golden_scene(tracer, name):
     assert srgb_lsb_diff(sh["xyz"].cpu().numpy(), xyz_c, e_c) <= 1
     for k in ("captured", "disc_hits", "sun_hits", "star_hits", "escaped", "jet_rays"):
         assert abs(getattr(st, k) - META["scenes"][name]["stats"][k]) <= rep["exempt"], k
+
+
+def test_display_transform_matches_reference(tracer):
+    """GPU auto exposure and develop against the reference build's own outputs
+    on 4000 pixels (lognormal XYZ with zeros): exposure bitwise, every tone
+    curve within 1 LSB (exactly equal where the curve has no transcendental
+    evaluated differently by the device libm)."""
+    import torch
+    xyz = torch.from_numpy(np.ascontiguousarray(ARR["display/xyz"])).cuda()
+    e = tracer.auto_exposure(xyz)
+    assert e == META["display"]["exposure"]
+    for tm in range(5):
+        rgb = tracer.develop(xyz, e, tm, 16.0).cpu().numpy().astype(int)
+        ref = ARR[f"display/rgb{tm}"].astype(int)
+        d = np.abs(rgb - ref)
+        print(f"[display tm={tm}] max {d.max()} LSB, {int((d > 0).sum())} channels differ", flush=True)
+        assert d.max() <= 1, tm
+
+
+def test_colour_table_matches_reference(tracer):
+    """spec::blackbody_xyz through the device table vs the reference's values
+    (same table, built on the host; the device log10 may differ by an ulp)."""
+    import torch
+    T = torch.from_numpy(np.ascontiguousarray(ARR["bb/T"])).cuda()
+    out = tracer.blackbody_xyz(T).cpu().numpy()
+    ref = ARR["bb/xyz"]
+    rel = np.abs(out - ref) / np.maximum(np.abs(ref), 1e-300)
+    print(f"[colour table] max rel {rel.max():.3e}", flush=True)
+    assert rel.max() < 1e-12
