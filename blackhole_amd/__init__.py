@@ -14,12 +14,12 @@ from . import abi, scene
 from .abi import CLASS_NAMES, Scene, Stats
 from .native import BhxError
 
-__all__ = ["abi", "scene", "Scene", "Stats", "CLASS_NAMES", "BhxError", "Tracer"]
+__all__ = ["abi", "scene", "Scene", "Stats", "CLASS_NAMES", "BhxError", "Tracer", "Batch"]
 
 
 def __getattr__(name):
     # Tracer imports torch lazily so that `import blackhole_amd` stays cheap.
-    if name == "Tracer":
-        from .tracer import Tracer
-        return Tracer
+    if name in ("Tracer", "Batch"):
+        from . import tracer
+        return getattr(tracer, name)
     raise AttributeError(name)

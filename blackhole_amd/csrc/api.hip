@@ -1,9 +1,5 @@
 // api.hip - the C ABI of include/bhx.h: context, uploads, launches.
-#include "../../include/bhx.h"
-#include "host_setup.h"
-#include "params.h"
-
-#include <hip/hip_runtime.h>
+#include "context.h"
 
 #include <algorithm>
 #include <chrono>
@@ -13,95 +9,11 @@
 #include <string>
 #include <vector>
 
-namespace bhx {
-hipError_t launch_raygen(const RaygenArgs& args, hipStream_t stream);
-hipError_t launch_trace(const TraceArgs& args, int grid, int block, hipStream_t stream);
-hipError_t trace_occupancy(int block, int* blocks_per_cu);
-hipError_t launch_shade(const ShadeArgs& args, int blocks, hipStream_t s);
-hipError_t launch_develop(const double* xyz, int64_t n, double exposure, int tm, double log_k,
-                          uint8_t* rgb, hipStream_t s);
-hipError_t launch_lum_hist(const double* xyz, int64_t n, int shift, unsigned long long prefix,
-                           unsigned int* hist, unsigned long long* n_pos, hipStream_t s);
-hipError_t launch_bb(const double* tab, const double* T, int64_t n, double* out, hipStream_t s);
-hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);
-hipError_t launch_zero_u32(unsigned int* p, int n, hipStream_t s);
-}  // namespace bhx
-
 using namespace bhx;
-
-// Small per-context device block: counters of one trace + one shade.
-struct Counters {
-    unsigned long long totals[4];       // steps, attempts, max norm bits, max Carter bits
-    unsigned long long classes[8];      // 6 classes + jet rays
-    unsigned int tile_counter;
-    unsigned int pad;
-};
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    hipError_t ensure(size_t n) {
-        if (n <= bytes) return hipSuccess;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        hipError_t e = hipMalloc(&p, n);
-        if (e == hipSuccess) bytes = n;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-};
-
-struct bhx_context {
-    int device = 0;
-    int n_cu = 0;
-    std::string err;
-    bhx_launch launch{256, 0, 64, 16};
-    // tables
-    double* d_bb = nullptr;
-    // catalogue (cell CSR + star rows)
-    bool custom_cat = false;
-    host::Catalogue custom;
-    bool cat_valid = false;
-    int cat_count = -1;
-    uint64_t cat_seed = 0;
-    bool cat_is_custom = false;
-    DevBuf cell_start, star_rows;
-    // queue order cache: tiles centre-first, first queue slot of each tile
-    int tiles_w = -1, tiles_h = -1, tiles_S = -1;
-    DevBuf tile_order, tile_start;
-    int n_tiles = 0;
-    // initial ray states in queue order (kInitRows x n) and slot -> ray map
-    DevBuf init, q2ray;
-    // jet / Sun parameters of the tracer's event branches (last uploaded copy)
-    DevBuf evp;
-    EventParams ev_host{};
-    bool ev_valid = false;
-    // counters
-    Counters* d_cnt = nullptr;
-    DevBuf flux_partial, lum_hist, lum_npos;
-    // frame buffers of bhx_render
-    DevBuf r_cls, r_y, r_jet, r_xyz;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-};
 
 namespace {
 
-int fail(bhx_context* ctx, int code, const std::string& msg) {
-    if (ctx) ctx->err = msg;
-    return code;
-}
-
-#define HIPCHK(call)                                                                      \
-    do {                                                                                  \
-        hipError_t e_ = (call);                                                           \
-        if (e_ != hipSuccess)                                                             \
-            return fail(ctx, BHX_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
-    } while (0)
+int fail(bhx_context* ctx, int code, const std::string& msg) { return bhx_fail(ctx, code, msg); }
 
 int check_scene(bhx_context* ctx, const bhx_scene* s) {
     if (!s) return fail(ctx, BHX_E_INVALID, "scene is NULL");
@@ -116,30 +28,31 @@ int check_scene(bhx_context* ctx, const bhx_scene* s) {
 // Centre-first order of 8 x 8 tiles: the photon ring (longest rays) sits near
 // the optical axis, so starting there shortens the queue's tail.
 int ensure_tiles(bhx_context* ctx, int W, int H, int S) {
-    const int tw = (W + 7) / 8, th = (H + 7) / 8;
-    if (tw == ctx->tiles_w && th == ctx->tiles_h && S == ctx->tiles_S) return BHX_OK;
-    const int n = tw * th;
-    std::vector<int32_t> order(n);
-    std::vector<double> key(n);
-    for (int i = 0; i < n; ++i) {
-        order[i] = i;
-        const double cx = (i % tw) * 8 + 4 - 0.5 * W, cy = (i / tw) * 8 + 4 - 0.5 * H;
-        key[i] = cx * cx + cy * cy;
+    const auto key = std::make_tuple(W, H, S);
+    auto it = ctx->tiles.find(key);
+    if (it != ctx->tiles.end()) {
+        ctx->cur_tiles = &it->second;
+        return BHX_OK;
     }
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
-    std::vector<int32_t> start(n + 1, 0);
-    for (int t = 0; t < n; ++t) {
-        const int tx = order[t] % tw, ty = order[t] / tw;
-        start[t + 1] = start[t] + std::min(8, W - tx * 8) * std::min(8, H - ty * 8) * S * S;
+    if (ctx->tiles.size() >= 256) {   // bound the cache: drain the device, then start over
+        HIPCHK(hipDeviceSynchronize());
+        for (auto& kv : ctx->tiles) {
+            kv.second.order.release();
+            kv.second.start.release();
+        }
+        ctx->tiles.clear();
     }
-    HIPCHK(ctx->tile_order.ensure(sizeof(int32_t) * n));
-    HIPCHK(hipMemcpy(ctx->tile_order.p, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
-    HIPCHK(ctx->tile_start.ensure(sizeof(int32_t) * (n + 1)));
-    HIPCHK(hipMemcpy(ctx->tile_start.p, start.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice));
-    ctx->tiles_w = tw;
-    ctx->tiles_h = th;
-    ctx->tiles_S = S;
-    ctx->n_tiles = n;
+    std::vector<int32_t> order, start;
+    host::tile_queue(W, H, S, 0, order, start);
+    const int n = static_cast<int>(order.size());
+    bhx_context::TileSet& t = ctx->tiles[key];
+    HIPCHK(t.order.ensure(sizeof(int32_t) * n));
+    HIPCHK(hipMemcpy(t.order.p, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    HIPCHK(t.start.ensure(sizeof(int32_t) * (n + 1)));
+    HIPCHK(hipMemcpy(t.start.p, start.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice));
+    t.tiles_x = (W + 7) / 8;
+    t.n_tiles = n;
+    ctx->cur_tiles = &t;
     return BHX_OK;
 }
 
@@ -235,7 +148,11 @@ int bhx_destroy(bhx_context* ctx) {
     (void)hipDeviceSynchronize();
     if (ctx->d_bb) (void)hipFree(ctx->d_bb);
     if (ctx->d_cnt) (void)hipFree(ctx->d_cnt);
-    for (DevBuf* b : {&ctx->cell_start, &ctx->star_rows, &ctx->tile_order, &ctx->tile_start,
+    for (auto& kv : ctx->tiles) {
+        kv.second.order.release();
+        kv.second.start.release();
+    }
+    for (DevBuf* b : {&ctx->cell_start, &ctx->star_rows,
                       &ctx->init, &ctx->q2ray, &ctx->evp, &ctx->flux_partial,
                       &ctx->lum_hist, &ctx->lum_npos, &ctx->r_cls, &ctx->r_y, &ctx->r_jet, &ctx->r_xyz})
         b->release();
@@ -301,10 +218,10 @@ int bhx_trace_dev(bhx_context* ctx, const bhx_scene* s, const bhx_rays* rays, bh
     HIPCHK(ctx->init.ensure(sizeof(double) * kInitRows * n_rays));
     HIPCHK(ctx->q2ray.ensure(sizeof(int32_t) * n_rays));
     RaygenArgs G = host::raygen_args(*s);
-    G.tiles_x = ctx->tiles_w;
-    G.n_tiles = ctx->n_tiles;
-    G.tile_order = static_cast<const int32_t*>(ctx->tile_order.p);
-    G.tile_start = static_cast<const int32_t*>(ctx->tile_start.p);
+    G.tiles_x = ctx->cur_tiles->tiles_x;
+    G.n_tiles = ctx->cur_tiles->n_tiles;
+    G.tile_order = static_cast<const int32_t*>(ctx->cur_tiles->order.p);
+    G.tile_start = static_cast<const int32_t*>(ctx->cur_tiles->start.p);
     G.init = static_cast<double*>(ctx->init.p);
     G.q2ray = static_cast<int32_t*>(ctx->q2ray.p);
     TraceArgs P = host::trace_args(*s);

@@ -1,0 +1,106 @@
+// context.h - internal: the context object behind the C ABI and the kernel
+// launchers, shared by api.hip (single frames) and batch.hip (batches).
+#pragma once
+
+#include "../../include/bhx.h"
+#include "host_setup.h"
+#include "params.h"
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <tuple>
+
+namespace bhx {
+hipError_t launch_raygen(const RaygenArgs& args, hipStream_t stream);
+hipError_t launch_trace(const TraceArgs& args, int grid, int block, hipStream_t stream);
+hipError_t trace_occupancy(int block, int* blocks_per_cu);
+hipError_t launch_trace_batch(const TraceArgs& args, int grid, int block, hipStream_t stream);
+hipError_t trace_batch_occupancy(int block, int n_scenes, int* blocks_per_cu);
+hipError_t launch_shade(const ShadeArgs& args, int blocks, hipStream_t s);
+hipError_t launch_develop(const double* xyz, int64_t n, double exposure, int tm, double log_k,
+                          uint8_t* rgb, hipStream_t s);
+hipError_t launch_lum_hist(const double* xyz, int64_t n, int shift, unsigned long long prefix,
+                           unsigned int* hist, unsigned long long* n_pos, hipStream_t s);
+hipError_t launch_bb(const double* tab, const double* T, int64_t n, double* out, hipStream_t s);
+hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);
+hipError_t launch_zero_u32(unsigned int* p, int n, hipStream_t s);
+}  // namespace bhx
+
+// Small per-context device block: counters of one trace + one shade.
+struct Counters {
+    unsigned long long totals[4];       // steps, attempts, max norm bits, max Carter bits
+    unsigned long long classes[8];      // 6 classes + jet rays
+    unsigned int tile_counter;
+    unsigned int pad;
+};
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t n) {
+        if (n <= bytes) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) bytes = n;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+struct bhx_context {
+    int device = 0;
+    int n_cu = 0;
+    std::string err;
+    bhx_launch launch{256, 0, 64, 16};
+    // tables
+    double* d_bb = nullptr;
+    // catalogue (cell CSR + star rows)
+    bool custom_cat = false;
+    bhx::host::Catalogue custom;
+    bool cat_valid = false;
+    int cat_count = -1;
+    uint64_t cat_seed = 0;
+    bool cat_is_custom = false;
+    DevBuf cell_start, star_rows;
+    // queue order cache, one entry per frame shape (W, H, S): tiles
+    // centre-first and the first queue slot of each tile.  Entries are never
+    // rewritten while they may be read by a launch in flight.
+    struct TileSet {
+        DevBuf order, start;
+        int tiles_x = 0, n_tiles = 0;
+    };
+    std::map<std::tuple<int, int, int>, TileSet> tiles;
+    const TileSet* cur_tiles = nullptr;
+    // initial ray states in queue order (kInitRows x n) and slot -> ray map
+    DevBuf init, q2ray;
+    // jet / Sun parameters of the tracer's event branches (last uploaded copy)
+    DevBuf evp;
+    bhx::EventParams ev_host{};
+    bool ev_valid = false;
+    // counters
+    Counters* d_cnt = nullptr;
+    DevBuf flux_partial, lum_hist, lum_npos;
+    // frame buffers of bhx_render
+    DevBuf r_cls, r_y, r_jet, r_xyz;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+inline int bhx_fail(bhx_context* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define HIPCHK(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return bhx_fail(ctx, BHX_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)

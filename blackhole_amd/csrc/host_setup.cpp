@@ -268,6 +268,25 @@ EventParams event_params(const bhx_scene& s) {
     return E;
 }
 
+void tile_queue(int W, int H, int S, int32_t q_base, std::vector<int32_t>& order,
+                std::vector<int32_t>& start) {
+    const int tw = (W + 7) / 8, th = (H + 7) / 8;
+    const int n = tw * th;
+    order.resize(n);
+    std::vector<double> key(n);
+    for (int i = 0; i < n; ++i) {
+        order[i] = i;
+        const double cx = (i % tw) * 8 + 4 - 0.5 * W, cy = (i / tw) * 8 + 4 - 0.5 * H;
+        key[i] = cx * cx + cy * cy;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+    start.assign(n + 1, q_base);
+    for (int t = 0; t < n; ++t) {
+        const int tx = order[t] % tw, ty = order[t] / tw;
+        start[t + 1] = start[t] + std::min(8, W - tx * 8) * std::min(8, H - ty * 8) * S * S;
+    }
+}
+
 TraceArgs trace_args(const bhx_scene& s) {
     TraceArgs P{};
     const double a = s.spin;

@@ -39,6 +39,13 @@ EventParams event_params(const bhx_scene& s);
 TraceArgs trace_args(const bhx_scene& s);
 ShadeArgs shade_args(const bhx_scene& s);
 
+// Work-queue order of a W x H frame with S x S samples per pixel: 8 x 8-pixel
+// tiles sorted by distance of their centre from the image centre (the photon
+// ring's long rays start first), and the first queue slot of each queue tile
+// (n_tiles + 1 entries, slot 0 = q_base).
+void tile_queue(int W, int H, int S, int32_t q_base, std::vector<int32_t>& order,
+                std::vector<int32_t>& start);
+
 // Kerr helpers used by the set-up code.
 double isco_radius(double a, int sense);
 double horizon_outer(double a);

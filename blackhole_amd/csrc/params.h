@@ -13,6 +13,17 @@ struct EventParams {
     double sun_c[3], sun_R;
 };
 
+// One scene of a batched launch: the tracer's per-scene constants (the
+// uniform fields of TraceArgs) plus its event parameters.  The batched tracer
+// copies the table into LDS and each lane reads its own scene's row.
+struct SceneK {
+    double a, a2, r_h, r_stop, rtol, atol, r_escape, h0, r_in, r_out, ns_R;
+    int32_t has_disc, has_ns, has_jet, has_sun, max_steps, pad;
+    EventParams ev;
+};
+static_assert(sizeof(SceneK) % sizeof(double) == 0, "SceneK is copied as doubles");
+constexpr int kMaxBatchScenes = 128;
+
 // Ray generation: camera, jitter and the initial photon of every sample
 // (src/render.cpp:437-452, src/scene.h:356-384, src/geodesic.h:215-231),
 // written in work-queue order so the tracer's refills are coalesced loads.
@@ -30,9 +41,13 @@ struct RaygenArgs {
     int32_t tiles_x, n_tiles;
     const int32_t* tile_order;                // queue tile -> image tile (8 x 8 pixels)
     const int32_t* tile_start;                // queue tile -> first queue slot
-    int32_t n_rays;
+    int32_t n_rays;                           // row stride of init (all rays of the launch)
     double* init;                             // kInitRows x n_rays, queue order
     int32_t* q2ray;                           // queue slot -> ray index
+    // batched launches: this scene's first ray index, its id, and the
+    // queue-slot -> scene map (NULL for a single scene)
+    int32_t ray_base, scene_id;
+    uint8_t* q2scene;
 };
 
 // Rows of the initial-state array.
@@ -53,6 +68,11 @@ struct TraceArgs {
     const double* init;             // kInitRows x n_rays, queue order
     const int32_t* q2ray;
     uint32_t* counter;              // next queue slot
+    // batched launch: per-scene constants (n_scenes rows) and the queue-slot
+    // -> scene map; the uniform fields above are then unused
+    const SceneK* scenes;
+    const uint8_t* q2scene;
+    int32_t n_scenes;
     // outputs (SoA over ray index, see bhx.h)
     uint8_t* cls;
     double* y;
@@ -70,7 +90,8 @@ struct TraceArgs {
 struct ShadeArgs {
     double a, a2, psf_sigma, observer_time;
     int32_t W, H, S, n_pix;
-    int64_t n_rays;
+    int64_t n_rays;                    // row stride of the SoA ray arrays
+    int64_t ray_base;                  // first ray of this scene in those arrays
     // Novikov-Thorne disc
     int32_t has_disc, disc_sg;
     double disc_a, r_in, r_out, flux_scale;   // flux_scale = mdot c^2 / r_g^2
@@ -107,6 +128,14 @@ struct ShadeArgs {
     // per-block partials: class counts [6] as u64 and flux sums
     unsigned long long* class_counts;   // 7 counters: 6 classes + jet rays
     double* flux_partial;               // one per block
+    // optional per-ray trace counters to reduce (batched launches, whose
+    // tracer cannot total per scene): accepted steps, attempts, norm and
+    // Carter maxima -> trace_totals[4] (sums, and maxima as bits)
+    const int32_t* steps_in;
+    const int32_t* attempts_in;
+    const double* norm_in;
+    const double* carter_in;
+    unsigned long long* trace_totals;
 };
 
 }  // namespace bhx

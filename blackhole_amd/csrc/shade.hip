@@ -121,18 +121,28 @@ __device__ void sky_radiance(const ShadeArgs& P, const double d[3], double o[3])
 __global__ void __launch_bounds__(256) k_shade(const ShadeArgs P) {
     __shared__ unsigned long long counts[7];
     __shared__ double flux_w[4];
+    __shared__ unsigned long long ttot[4];
     if (threadIdx.x < 7) counts[threadIdx.x] = 0;
+    if (threadIdx.x < 4) ttot[threadIdx.x] = 0;
     __syncthreads();
     const int pix = blockIdx.x * blockDim.x + threadIdx.x;
     const int S2 = P.S * P.S;
     const int64_t N = P.n_rays;
     unsigned int my_counts[7] = {0, 0, 0, 0, 0, 0, 0};
     double my_flux = 0.0;
+    unsigned long long my_steps = 0, my_att = 0;
+    double my_norm = 0.0, my_carter = 0.0;
     if (pix < P.n_pix) {
         double acc[3] = {0.0, 0.0, 0.0};
         for (int k = 0; k < S2; ++k) {
-            const int64_t ray = (int64_t)pix * S2 + k;
+            const int64_t ray = P.ray_base + (int64_t)pix * S2 + k;
             const int cls = P.cls[ray];
+            if (P.steps_in) {
+                my_steps += (unsigned long long)P.steps_in[ray];
+                my_att += (unsigned long long)P.attempts_in[ray];
+                my_norm = dmax(my_norm, P.norm_in[ray]);
+                my_carter = dmax(my_carter, P.carter_in[ray]);
+            }
             double y[8];
 #pragma unroll
             for (int c = 0; c < 8; ++c) y[c] = P.y[c * N + ray];
@@ -283,8 +293,28 @@ __global__ void __launch_bounds__(256) k_shade(const ShadeArgs P) {
     }
     for (int off = 32; off > 0; off >>= 1) my_flux += __shfl_xor(my_flux, off);
     if (__lane_id() == 0) flux_w[threadIdx.x / 64] = my_flux;
+    if (P.steps_in) {
+        for (int off = 32; off > 0; off >>= 1) {
+            my_steps += __shfl_xor(my_steps, off);
+            my_att += __shfl_xor(my_att, off);
+            my_norm = dmax(my_norm, __shfl_xor(my_norm, off));
+            my_carter = dmax(my_carter, __shfl_xor(my_carter, off));
+        }
+        if (__lane_id() == 0) {
+            atomicAdd(&ttot[0], my_steps);
+            atomicAdd(&ttot[1], my_att);
+            atomicMax(&ttot[2], (unsigned long long)__double_as_longlong(my_norm));  // >= 0: orders as bits
+            atomicMax(&ttot[3], (unsigned long long)__double_as_longlong(my_carter));
+        }
+    }
     __syncthreads();
     if (threadIdx.x < 7 && counts[threadIdx.x]) atomicAdd(&P.class_counts[threadIdx.x], counts[threadIdx.x]);
+    if (P.steps_in && threadIdx.x == 0) {
+        atomicAdd(&P.trace_totals[0], ttot[0]);
+        atomicAdd(&P.trace_totals[1], ttot[1]);
+        atomicMax(&P.trace_totals[2], ttot[2]);
+        atomicMax(&P.trace_totals[3], ttot[3]);
+    }
     if (threadIdx.x == 0) {
         double s = 0.0;
         for (int w = 0; w < (int)(blockDim.x / 64); ++w) s += flux_w[w];

@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(64) k_raygen(const RaygenArgs P) {
         const int pin = i / S2, k = i - pin * S2;
         const int px = x0 + pin % tw, py = y0 + pin / tw;
         const int q = base + i;
-        const int ray = (py * P.W + px) * S2 + k;
+        const int ray = P.ray_base + (py * P.W + px) * S2 + k;
         // jitter (render.cpp:440-446): draws 2k+1, 2k+2 of the pixel's stream
         const uint64_t sd = P.seed ^ ((uint64_t)py << 32) ^ ((uint64_t)px * 0x9E3779B97F4A7C15ull);
         const int sx = k % P.S, sy = k / P.S;
@@ -212,17 +212,26 @@ __global__ void __launch_bounds__(64) k_raygen(const RaygenArgs P) {
         I[kI_C0 * N + q] = f.cs;
         I[kI_Q0 * N + q] = q0;
         P.q2ray[q] = ray;
+        if (P.q2scene) P.q2scene[q] = (uint8_t)P.scene_id;
     }
 }
 
 // ---------------------------------------------------------------------------
 // The persistent tracer.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-k_trace(const TraceArgs P) {
-    __shared__ unsigned long long blk[4];  // steps, attempts, max norm bits, max Carter bits
-    if (threadIdx.x < 4) blk[threadIdx.x] = 0ull;
-    __syncthreads();
+// The tracer body, shared by the single-scene kernel (kBatch = false: every
+// scene constant is a kernel argument, uniform, in scalar registers) and the
+// batched kernel (kBatch = true: each lane's ray belongs to its own scene; the
+// spin and tolerances live in the lane's registers, everything else in the
+// LDS copy `sk` of the scene table).
+#define SK(f) (kBatch ? sk[sid].f : P.f)
+#define EV (kBatch ? sk[sid].ev : *P.ev)
+template <bool kBatch>
+__device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
+                                           unsigned long long* blk) {
+    // ---- per-scene constants -------------------------------------------------
+    double la = P.a, la2 = P.a2, lrtol = P.rtol, latol = P.atol;
+    int sid = 0;
 
     // ---- lane state ------------------------------------------------------
     bool live = false;
@@ -260,6 +269,13 @@ k_trace(const TraceArgs P) {
                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
                 if (!live && rank < avail) {
                     q = c_next + rank;
+                    if constexpr (kBatch) {
+                        sid = P.q2scene[q];
+                        la = sk[sid].a;
+                        la2 = sk[sid].a2;
+                        lrtol = sk[sid].rtol;
+                        latol = sk[sid].atol;
+                    }
                     const double* I = P.init;
 #pragma unroll
                     for (int c = 0; c < 6; ++c) {
@@ -271,11 +287,11 @@ k_trace(const TraceArgs P) {
                     c_prev = I[kI_C0 * N + q];
                     q0raw = I[kI_Q0 * N + q];
                     iq0 = rcp(fabs(q0raw) + 1e-12);
-                    h = P.h0;
+                    h = SK(h0);
                     pw = P.pw_init;
                     nsteps = natt = tries = 0;
                     mx_norm = mx_carter = jet_acc = 0.0;
-                    live = P.max_steps > 0;
+                    live = SK(max_steps) > 0;
                     if (!live) {  // the step loop never runs: Exhausted at y0
                         const int ray = P.q2ray[q];
                         if (P.cls) P.cls[ray] = BHX_EXHAUSTED;
@@ -303,11 +319,11 @@ k_trace(const TraceArgs P) {
 
         // ---- one DP5 attempt (geodesic.h:130-194) ---------------------------
         {
-            const double hmax = 0.05 * dmax(1.0, y[1] - P.r_h) + 0.01;
+            const double hmax = 0.05 * dmax(1.0, y[1] - SK(r_h)) + 0.01;
             h = (tries == 0 && h > hmax) ? hmax : h;   // clamp once per step, not per retry
         }
         double tmp[6], k2[6], k3[6], k4[6], k5[6], k6[6], yn[6];
-        const RayK K = ray_k(P.a, P.a2, pt, pph);
+        const RayK K = ray_k(la, la2, pt, pph);
         Rhs f;
 #pragma unroll
         for (int c = 0; c < 6; ++c) tmp[c] = y[c] + h * A21 * k1[c];
@@ -345,7 +361,7 @@ k_trace(const TraceArgs P) {
         for (int c = 0; c < 6; ++c) {
             const double e = h * (E1 * k1[c] + E3 * k3[c] + E4 * k4[c] + E5 * k5[c] + E6 * k6[c] +
                                   E7 * g7.d[c]);
-            const double sc = P.atol + P.rtol * dmax(fabs(y[c]), fabs(yn[c]));
+            const double sc = latol + lrtol * dmax(fabs(y[c]), fabs(yn[c]));
             const double qe = e * rcp(sc);
             err2 += qe * qe;   // the p_t, p_phi terms are exactly 0
             c5[c] = h * (D1 * k1[c] + D3 * k3[c] + D4 * k4[c] + D5 * k5[c] + D6 * k6[c] +
@@ -369,35 +385,35 @@ k_trace(const TraceArgs P) {
                 // the five terms of g^{ab} p_a p_b (kerr.h:301-321) from the FSAL geometry
                 const double p1 = yn[4], p2 = yn[5], rn = yn[1];
                 const double iQ = g7.iS * g7.iD;
-                const double A = g7.rpa * g7.rpa - P.a2 * g7.D * g7.sn2;
+                const double A = g7.rpa * g7.rpa - la2 * g7.D * g7.sn2;
                 const double t0 = -A * iQ * K.pt * K.pt;
-                const double t1 = -4.0 * P.a * rn * iQ * K.pt * K.pph;
+                const double t1 = -4.0 * la * rn * iQ * K.pt * K.pph;
                 const double t2 = g7.D * g7.iS * p1 * p1, t3 = g7.iS * p2 * p2;
-                const double t4 = (g7.D - P.a2 * g7.sn2) * iQ * g7.isn2 * K.pph2;
+                const double t4 = (g7.D - la2 * g7.sn2) * iQ * g7.isn2 * K.pph2;
                 const double nrm = t0 + t1 + t2 + t3 + t4;
                 const double sc = fabs(t0) + fabs(t1) + fabs(t2) + fabs(t3) + fabs(t4);
                 const double rel = (sc > 0.0) ? nrm * rcp(sc) : nrm;
                 mx_norm = dmax(mx_norm, fabs(rel));
                 const double E = -pt;
-                const double Q = p2 * p2 + g7.cs * g7.cs * (P.a2 * (0.0 - E * E) + pph * pph * g7.isn2);
+                const double Q = p2 * p2 + g7.cs * g7.cs * (la2 * (0.0 - E * E) + pph * pph * g7.isn2);
                 mx_carter = dmax(mx_carter, fabs(Q - q0raw) * iq0);
             }
-            const bool ns_hit = P.has_ns && yn[1] <= P.ns_R;
-            const bool horizon = !ns_hit && yn[1] <= P.r_stop;
-            const bool disc_cross = !ns_hit && !horizon && P.has_disc && c_prev * g7.cs < 0.0;
+            const bool ns_hit = SK(has_ns) && yn[1] <= SK(ns_R);
+            const bool horizon = !ns_hit && yn[1] <= SK(r_stop);
+            const bool disc_cross = !ns_hit && !horizon && SK(has_disc) && c_prev * g7.cs < 0.0;
             // jet cull and Sun candidate need the pseudo-Cartesian end points of the step
             bool near_jet = false, sun_cand = false;
             double p0[3], p1[3];
-            if (P.has_jet || P.has_sun) {
-                to_cart(P.a2, y[1], y[2], y[3], p0);
-                to_cart(P.a2, yn[1], yn[2], yn[3], p1);
-                if (P.has_jet) near_jet = seg_dist(p0, p1) < P.ev->r_bound;          // render.cpp:151-152
-                if (P.has_sun && !ns_hit && !horizon) {                             // render.cpp:203-209
-                    const double* C = P.ev->sun_c;
+            if (SK(has_jet) || SK(has_sun)) {
+                to_cart(la2, y[1], y[2], y[3], p0);
+                to_cart(la2, yn[1], yn[2], yn[3], p1);
+                if (SK(has_jet)) near_jet = seg_dist(p0, p1) < EV.r_bound;          // render.cpp:151-152
+                if (SK(has_sun) && !ns_hit && !horizon) {                             // render.cpp:203-209
+                    const double* C = EV.sun_c;
                     const double e0 = p1[0] - p0[0], e1 = p1[1] - p0[1], e2 = p1[2] - p0[2];
                     const double seg_len = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
                     const double f0 = p0[0] - C[0], f1 = p0[1] - C[1], f2 = p0[2] - C[2];
-                    sun_cand = sqrt(f0 * f0 + f1 * f1 + f2 * f2) <= seg_len + P.ev->sun_R * 1.5;
+                    sun_cand = sqrt(f0 * f0 + f1 * f1 + f2 * f2) <= seg_len + EV.sun_R * 1.5;
                 }
             }
             double s_jet = -1.0;   // fraction of the step the jet integral covers
@@ -420,7 +436,7 @@ k_trace(const TraceArgs P) {
                 }
                 if (ns_hit) {
                     // ---- neutron-star surface (render.cpp:158-169) -------------
-                    const double se = root_r(c1[1], c2[1], c3[1], c4[1], c5[1], P.ns_R, y[1], yn[1]);
+                    const double se = root_r(c1[1], c2[1], c3[1], c4[1], c5[1], SK(ns_R), y[1], yn[1]);
 #pragma unroll
                     for (int c = 0; c < 6; ++c) yo[c] = dense_comp(c1[c], c2[c], c3[c], c4[c], c5[c], se);
                     term = BHX_STAR;
@@ -430,7 +446,7 @@ k_trace(const TraceArgs P) {
                     // ---- equatorial crossing (render.cpp:180-199) ---------------
                     const double se = root_theta(c1[2], c2[2], c3[2], c4[2], c5[2], c_prev, y[2], yn[2]);
                     const double rc = dense_comp(c1[1], c2[1], c3[1], c4[1], c5[1], se);
-                    if (rc >= P.r_in && rc <= P.r_out) {
+                    if (rc >= SK(r_in) && rc <= SK(r_out)) {
 #pragma unroll
                         for (int c = 0; c < 6; ++c) yo[c] = dense_comp(c1[c], c2[c], c3[c], c4[c], c5[c], se);
                         term = BHX_DISC;
@@ -439,11 +455,11 @@ k_trace(const TraceArgs P) {
                 }
                 if (term < 0 && sun_cand) {
                     // ---- the Sun: 12 probes then 50 bisections (render.cpp:210-240)
-                    const double* C = P.ev->sun_c;
-                    const double R2 = P.ev->sun_R * P.ev->sun_R;
+                    const double* C = EV.sun_c;
+                    const double R2 = EV.sun_R * EV.sun_R;
                     auto fsun = [&](double sv) {
                         double q[3];
-                        to_cart(P.a2, dense_comp(c1[1], c2[1], c3[1], c4[1], c5[1], sv),
+                        to_cart(la2, dense_comp(c1[1], c2[1], c3[1], c4[1], c5[1], sv),
                                 dense_comp(c1[2], c2[2], c3[2], c4[2], c5[2], sv),
                                 dense_comp(c1[3], c2[3], c3[3], c4[3], c5[3], sv), q);
                         const double dx = q[0] - C[0], dy = q[1] - C[1], dz = q[2] - C[2];
@@ -476,7 +492,7 @@ k_trace(const TraceArgs P) {
                 if (term < 0) s_jet = 1.0;   // nothing stopped the ray: the whole step counts
                 if (near_jet && s_jet > 0.0) {
                     // ---- jet emission: 4-point Gauss-Legendre (render.cpp:107-122)
-                    const EventParams& J = *P.ev;
+                    const EventParams& J = EV;
                     double sum = 0.0;
 #pragma unroll
                     for (int qd = 0; qd < 4; ++qd) {
@@ -485,9 +501,9 @@ k_trace(const TraceArgs P) {
                         const double ths = dense_comp(c1[2], c2[2], c3[2], c4[2], c5[2], sv);
                         double sn, cs;
                         sincos_near(ths, &sn, &cs);
-                        const double sh = jet_shape(J, rs * cs, sqrt(rs * rs + P.a2) * sn);
+                        const double sh = jet_shape(J, rs * cs, sqrt(rs * rs + la2) * sn);
                         if (sh <= 0.0) continue;
-                        const double b = jet_boost(J, P.a, P.a2, rs, ths, pt,
+                        const double b = jet_boost(J, la, la2, rs, ths, pt,
                                                    dense_comp(c1[4], c2[4], c3[4], c4[4], c5[4], sv),
                                                    dense_comp(c1[5], c2[5], c3[5], c4[5], c5[5], sv), pph);
                         if (b <= 0.0) continue;
@@ -496,7 +512,7 @@ k_trace(const TraceArgs P) {
                     jet_acc += J.j_nu0 * sum * h * s_jet;
                 }
             }
-            if (term < 0 && yn[1] > P.r_escape) {
+            if (term < 0 && yn[1] > SK(r_escape)) {
                 // ---- escape (render.cpp:247-251), after the jet sample ----------
 #pragma unroll
                 for (int c = 0; c < 6; ++c) yo[c] = yn[c];
@@ -508,7 +524,7 @@ k_trace(const TraceArgs P) {
                 c_prev = g7.cs;
                 pw = exp_near(kBeta * dmax(lg, kLog1em4));   // pow(max(err, 1e-4), beta)
                 h *= fac;
-                if (nsteps >= P.max_steps) {
+                if (nsteps >= SK(max_steps)) {
 #pragma unroll
                     for (int c = 0; c < 6; ++c) yo[c] = y[c];
                     term = BHX_EXHAUSTED;
@@ -546,7 +562,12 @@ k_trace(const TraceArgs P) {
         }
     }
 
-    // ---- frame totals: one global atomic per block ---------------------------
+}
+#undef SK
+#undef EV
+
+__device__ __forceinline__ void flush_totals(const TraceArgs& P, unsigned long long* blk) {
+    // frame totals: one global atomic per block
     __syncthreads();
     if (threadIdx.x == 0) {
         atomicAdd(&P.totals[0], blk[0]);
@@ -556,6 +577,32 @@ k_trace(const TraceArgs P) {
     }
 }
 
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_trace(const TraceArgs P) {
+    __shared__ unsigned long long blk[4];  // steps, attempts, max norm bits, max Carter bits
+    if (threadIdx.x < 4) blk[threadIdx.x] = 0ull;
+    __syncthreads();
+    trace_body<false>(P, nullptr, blk);
+    flush_totals(P, blk);
+}
+
+// Batched: the scene table (n_scenes rows) is copied into dynamic LDS first.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_trace_batch(const TraceArgs P) {
+    __shared__ unsigned long long blk[4];
+    extern __shared__ double sk_raw[];
+    SceneK* sk = reinterpret_cast<SceneK*>(sk_raw);
+    if (threadIdx.x < 4) blk[threadIdx.x] = 0ull;
+    {
+        const double* src = reinterpret_cast<const double*>(P.scenes);
+        const int nd = P.n_scenes * (int)(sizeof(SceneK) / sizeof(double));
+        for (int i = threadIdx.x; i < nd; i += blockDim.x) sk_raw[i] = src[i];
+    }
+    __syncthreads();
+    trace_body<true>(P, sk, blk);
+    flush_totals(P, blk);
+}
+
 }  // namespace
 
 hipError_t launch_raygen(const RaygenArgs& args, hipStream_t stream) {
@@ -563,14 +610,25 @@ hipError_t launch_raygen(const RaygenArgs& args, hipStream_t stream) {
     return hipGetLastError();
 }
 
-// Host launcher: one persistent grid.
+// Host launchers: one persistent grid.
 hipError_t launch_trace(const TraceArgs& args, int grid, int block, hipStream_t stream) {
     hipLaunchKernelGGL(k_trace, dim3(grid), dim3(block), 0, stream, args);
     return hipGetLastError();
 }
 
+hipError_t launch_trace_batch(const TraceArgs& args, int grid, int block, hipStream_t stream) {
+    const size_t lds = sizeof(SceneK) * (size_t)args.n_scenes;
+    hipLaunchKernelGGL(k_trace_batch, dim3(grid), dim3(block), lds, stream, args);
+    return hipGetLastError();
+}
+
 hipError_t trace_occupancy(int block, int* blocks_per_cu) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_trace, block, 0);
+}
+
+hipError_t trace_batch_occupancy(int block, int n_scenes, int* blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_trace_batch, block,
+                                                        sizeof(SceneK) * (size_t)n_scenes);
 }
 
 }  // namespace bhx

@@ -130,3 +130,39 @@ def neutron_star_scene(s, R_m, M_kg, *, frame=True):
         s.cam_fov = 5.0 * math.atan2(s.ns_R / math.sqrt(1.0 - 2.0 / s.ns_R), s.cam_r)
     s.r_escape = escape_radius(s)
     return s
+
+
+# The Gamma = 2, K = 100 polytrope of relativistic-star test problems (G = c =
+# M_sun = 1) in SI units for P = K rho^2, rho the total mass-energy density.
+G_SI, C_SI = 6.67430e-11, 2.99792458e8
+POLYTROPE_K100_SI = 100.0 * (G_SI * M_SUN / C_SI ** 2) ** 3 * C_SI ** 2 / M_SUN
+
+
+def survey_batch(seed=20240921, n=64, n_ns=8, side_min=128, side_max=1024):
+    """A seeded batch of uneven scenes: spin U[0, 0.998], inclination U[5, 85]
+    deg, observer radius log-U[100, 2000] M, square side log-U[side_min,
+    side_max], disc at log-U[0.01, 1] Eddington out to 30 M, image plane
+    +-25 M; n_ns of them hold instead a neutron star of mass U[1.2, 1.99]
+    M_sun on the K = 100 polytrope (hot caps, 200 Hz), framed by the CLI's
+    rule (src/main.cpp:514-519) at the drawn distance.  No star field."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    ns_idx = set(rng.choice(n, size=n_ns, replace=False).tolist()) if n_ns else set()
+    out = []
+    for i in range(n):
+        a = rng.uniform(0.0, 0.998)
+        incl = rng.uniform(5.0, 85.0)
+        r_cam = math.exp(rng.uniform(math.log(100.0), math.log(2000.0)))
+        side = int(round(math.exp(rng.uniform(math.log(side_min), math.log(side_max)))))
+        edd = math.exp(rng.uniform(math.log(0.01), math.log(1.0)))
+        m_ns = rng.uniform(1.2, 1.99)
+        s = plain_scene(spin=a, incl_deg=incl, r_cam=r_cam, width=side, height=side,
+                        fov=2.0 * math.atan(25.0 / r_cam), disc_r_out=30.0, eddington=edd)
+        if i in ns_idx:
+            R_m, M_kg = polytrope_star(POLYTROPE_K100_SI, 2.0, m_ns * M_SUN)
+            s.ns_spin_hz, s.ns_caps = 200.0, 1
+            neutron_star_scene(s, R_m, M_kg, frame=False)
+            b_app = s.ns_R / math.sqrt(1.0 - 2.0 / s.ns_R)
+            s.cam_fov = 5.0 * math.atan2(b_app, s.cam_r)
+        out.append(s)
+    return out

@@ -168,3 +168,76 @@ class Tracer:
         native.check(self.ctx, self.lib.bhx_blackbody_xyz_dev(self.ctx, _ptr(T), T.numel(), _ptr(out),
                                                               self._stream(stream)), "bhx_blackbody_xyz_dev")
         return out
+
+
+class Batch:
+    """Many scenes traced as one job (bhx_batch): one persistent tracer
+    launch over every scene's rays, replayed as a hipGraph."""
+
+    def __init__(self, tracer, scenes):
+        self.tr = tracer
+        self.lib = tracer.lib
+        self.scenes = list(scenes)
+        arr = (Scene * len(self.scenes))(*self.scenes)
+        self.b = C.c_void_p()
+        native.check(tracer.ctx, self.lib.bhx_batch_create(tracer.ctx, arr, len(self.scenes), C.byref(self.b)),
+                     "bhx_batch_create")
+
+    def close(self):
+        if self.b:
+            self.lib.bhx_batch_destroy(self.b)
+            self.b = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, graph=True, stream=None):
+        native.check(self.tr.ctx, self.lib.bhx_batch_run(self.b, int(bool(graph)), self.tr._stream(stream)),
+                     "bhx_batch_run")
+
+    def stats(self, stream=None):
+        """Per-scene Stats of the last run and the batch's device time (ms)."""
+        st = (Stats * len(self.scenes))()
+        ms = C.c_double()
+        native.check(self.tr.ctx, self.lib.bhx_batch_stats(self.b, st, C.byref(ms), self.tr._stream(stream)),
+                     "bhx_batch_stats")
+        return list(st), ms.value
+
+    def scene(self, i, stream=None):
+        """Host copies of scene i's results: cls, y (8, n), jet, steps, attempts, xyz (H, W, 3)."""
+        sc = self.scenes[i]
+        n = sc.n_rays
+        out = {"cls": np.zeros(n, np.uint8), "y": np.zeros((8, n)), "jet": np.zeros(n),
+               "steps": np.zeros(n, np.int32), "attempts": np.zeros(n, np.int32),
+               "xyz": np.zeros((sc.height, sc.width, 3))}
+        native.check(self.tr.ctx, self.lib.bhx_batch_copy_scene(
+            self.b, i, *[out[k].ctypes.data_as(C.c_void_p) for k in ("cls", "y", "jet", "steps", "attempts", "xyz")],
+            self.tr._stream(stream)), "bhx_batch_copy_scene")
+        return out
+
+    def set_launch(self, block=0, waves_per_simd=0, chunk=0, refill=0):
+        native.check(self.tr.ctx, self.lib.bhx_batch_set_launch(
+            self.b, C.byref(Launch(block, waves_per_simd, chunk, refill))), "bhx_batch_set_launch")
+
+    def get_launch(self):
+        l = Launch()
+        native.check(self.tr.ctx, self.lib.bhx_batch_get_launch(self.b, C.byref(l)), "bhx_batch_get_launch")
+        return {"block": l.block, "waves_per_simd": l.waves_per_simd, "chunk": l.chunk, "refill": l.refill}
+
+    def autotune(self, reps=2, stream=None):
+        """Pick the fastest launch shape for this batch; returns (shape, ms)."""
+        l = Launch()
+        ms = C.c_double()
+        native.check(self.tr.ctx, self.lib.bhx_batch_autotune(self.b, reps, C.byref(l), C.byref(ms),
+                                                              self.tr._stream(stream)), "bhx_batch_autotune")
+        return {"block": l.block, "waves_per_simd": l.waves_per_simd, "chunk": l.chunk,
+                "refill": l.refill}, ms.value

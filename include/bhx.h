@@ -18,6 +18,9 @@
  *   - Functions whose name ends in _dev take DEVICE pointers and a hipStream_t
  *     passed as void* (NULL = the null stream); they are stream-ordered and do
  *     not synchronise unless stated.  All other pointers are host pointers.
+ *   - A context's scratch (queue, initial states, counters) is reused by its
+ *     next call: issue one context's calls on one stream, or synchronise
+ *     between streams; use one context per host thread.
  *   - Ray index of sample (sx, sy) of pixel (x, y):
  *         ray = (y * width + x) * S*S + sy * S + sx,     S = sqrt_spp
  *     which is the order the reference visits samples (src/render.cpp:437-459).
@@ -203,6 +206,54 @@ typedef struct bhx_launch {
 
 int bhx_set_launch(bhx_context *ctx, const bhx_launch *launch);
 int bhx_get_launch(const bhx_context *ctx, bhx_launch *launch);
+
+/* ---- batches ------------------------------------------------------------
+ *
+ * Many scenes traced as one job: what a caller of the reference does with a
+ * loop of bh::render calls (src/render.h:72; the CLI's animation loop
+ * src/main.cpp:931-997 is one), done as ray generation per scene, ONE
+ * persistent tracer launch over every scene's rays (each lane reads its own
+ * scene's constants), and shading per scene, captured once into a hipGraph.
+ * Per-ray and per-pixel results are those of bhx_trace_dev / bhx_shade_dev
+ * on each scene alone.  Limits: at most 128 scenes and 2e9 samples per batch;
+ * scenes with stars must share star_count and seed (one catalogue).
+ */
+typedef struct bhx_batch bhx_batch;
+
+/* Copies the scenes and allocates every device buffer of the batch. */
+int bhx_batch_create(bhx_context *ctx, const bhx_scene *scenes, int32_t n, bhx_batch **out);
+int bhx_batch_destroy(bhx_batch *batch);
+
+/* Enqueue the whole batch on `stream`.  use_graph != 0: the first call
+ * captures the sequence into a hipGraph (on a private stream) and every call
+ * launches that graph; use_graph == 0: plain stream launches. */
+int bhx_batch_run(bhx_batch *batch, int32_t use_graph, void *stream);
+
+/* Device views of scene i after a run: the batch's per-ray arrays (ray j of
+ * the scene is index ray_base + j; component c of the hit state is at
+ * y[c * stride + ray_base + j]) and the scene's W*H*3 XYZ pixels. */
+int bhx_batch_scene(const bhx_batch *batch, int32_t i, bhx_rays *rays, int64_t *ray_base,
+                    int64_t *stride, double **xyz);
+
+/* Copy scene i's results to HOST buffers (any may be NULL; synchronises
+ * `stream`): class [n], hit state [8][n] component-major, jet [n], accepted
+ * steps [n], attempts [n] (n = the scene's samples) and XYZ [W*H*3]. */
+int bhx_batch_copy_scene(bhx_batch *batch, int32_t i, uint8_t *cls, double *y, double *jet,
+                         int32_t *steps, int32_t *attempts, double *xyz, void *stream);
+
+/* Per-scene RenderStats of the last run (n entries; synchronises `stream`);
+ * batch_ms = device time of the whole batch. */
+int bhx_batch_stats(bhx_batch *batch, bhx_stats *stats, double *batch_ms, void *stream);
+
+/* Launch shape of the batch's tracer (0 in a field keeps the value). */
+int bhx_batch_set_launch(bhx_batch *batch, const bhx_launch *launch);
+int bhx_batch_get_launch(const bhx_batch *batch, bhx_launch *launch);
+
+/* Autotuner: coordinate search over block size, occupancy, queue chunk and
+ * refill threshold, each candidate timed over `reps` graph replays of this
+ * batch; leaves the fastest shape set and returns it with its time. */
+int bhx_batch_autotune(bhx_batch *batch, int32_t reps, bhx_launch *best, double *best_ms,
+                       void *stream);
 
 /* ---- scene preparation (host only, no context, no GPU) ------------------ */
 
