@@ -12,12 +12,12 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libbhx.so")
 
-HIP_SOURCES = ["tracer.hip", "shade.hip", "api.hip", "batch.hip"]
+HIP_SOURCES = ["tracer.hip", "shade.hip", "api.hip", "batch.hip", "adjoint.hip"]
 # Per-file flags.  The tracer's loop uses ~70 distinct FP64 constants (DP5
 # tableau, sincos/log/exp coefficients); FP64 VALU ops take no 64-bit
 # literals, so machine LICM would pin every one of them in two registers for
 # the whole loop and spill.  Without it they are rematerialised where used.
-EXTRA = {"tracer.hip": ["-mllvm", "-disable-machine-licm"]}
+EXTRA = {"tracer.hip": ["-mllvm", "-disable-machine-licm"], "adjoint.hip": ["-mllvm", "-disable-machine-licm"]}
 CXX_SOURCES = ["host_setup.cpp", "host_physics.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

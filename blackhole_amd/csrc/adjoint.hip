@@ -1,0 +1,336 @@
+// adjoint.hip - the differentiable render on gfx950: a recording forward
+// kernel and the discrete-adjoint kernel over the per-ray pieces of
+// adjoint_dev.h, behind the bhx_adjoint_* entry points.
+//
+//   k_adj_forward   one ray per lane; a lane whose ray ends takes the next ray
+//                   index from a global counter in the same iteration, so every
+//                   iteration is one DP5 attempt on every lane.  Writes the
+//                   class, hit state, accepted-step count and bolometric
+//                   intensity of each sample, and a checkpoint of the
+//                   integrator state every K accepted steps (HBM slab of
+//                   cap = max_steps / K + 1 records of 8 doubles per ray).
+//   k_adj_backward  one disc-hit ray per lane, rays sorted by step count so the
+//                   lanes of a wave replay and sweep segments of equal length
+//                   together; each lane's K-step segment buffer is a strided
+//                   HBM scratch.  Per-block partial gradients, summed on the
+//                   host in a fixed order.
+#include "adjoint_frame.h"
+#include "context.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+using namespace bhx;
+using namespace bhx::adj;
+
+namespace {
+
+struct AdjArgs {
+    int64_t n_rays;
+    // forward
+    uint32_t* counter;
+    double* ck;               // n_rays * cap * kCk
+    uint8_t* cls;
+    double* y;                // 8 x n_rays SoA (layout of bhx_rays.y)
+    int32_t* steps;
+    double* I;                // n_rays bolometric intensity
+    // backward
+    const double* w;          // dL/dI per sample
+    const int32_t* order;     // disc-hit rays, most accepted steps first
+    int32_t n_order;
+    double* seg;              // K * 5 doubles per lane, strided by seg_stride
+    int64_t seg_stride;
+    double* partial;          // 4 per block
+    unsigned int* mismatch;
+};
+
+struct AdjConst {
+    Frame F;
+    RaygenArgs G;
+};
+
+__global__ void __launch_bounds__(256) k_adj_forward(const AdjConst* __restrict__ C, const AdjArgs P) {
+    const Frame& F = C->F;
+    const RaygenArgs& G = C->G;
+    Ray s;
+    bool live = false, done = false;
+    int64_t ray = 0;
+    const int64_t slab = static_cast<int64_t>(F.cap) * kCk;
+    for (;;) {
+        if (!live && !done) {
+            const uint32_t r = atomicAdd(P.counter, 1u);
+            if ((int64_t)r >= P.n_rays) {
+                done = true;
+            } else {
+                ray = r;
+                fwd_init(F, G, ray, s, P.ck + ray * slab);
+                live = true;
+            }
+        }
+        if (__ballot(live) == 0ull) break;
+        if (!live) continue;
+        double yo[6], se = 0.0;
+        const int term = fwd_step(F, s, P.ck + ray * slab, yo, &se);
+        if (term >= 0) {
+            const int64_t N = P.n_rays;
+            P.cls[ray] = (uint8_t)term;
+            P.y[ray] = yo[0]; P.y[N + ray] = yo[1]; P.y[2 * N + ray] = yo[2]; P.y[3 * N + ray] = yo[3];
+            P.y[4 * N + ray] = s.pt; P.y[5 * N + ray] = yo[4]; P.y[6 * N + ray] = yo[5]; P.y[7 * N + ray] = s.pph;
+            P.steps[ray] = s.nsteps;
+            P.I[ray] = (term == BHX_DISC) ? disc_intensity(F, yo[1], s.pt, s.pph) : 0.0;
+            live = false;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_adj_backward(const AdjConst* __restrict__ C, const AdjArgs P) {
+    __shared__ double part[4][4];
+    const Frame& F = C->F;
+    const RaygenArgs& G = C->G;
+    const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nl = (int64_t)gridDim.x * blockDim.x;
+    const int64_t slab = static_cast<int64_t>(F.cap) * kCk;
+    double g[4] = {0.0, 0.0, 0.0, 0.0};
+    unsigned int mism = 0;
+    for (int64_t i = lane; i < P.n_order; i += nl) {
+        const int64_t ray = P.order[i];
+        const double w = P.w[ray];
+        if (w == 0.0) continue;
+        mism += bwd_ray(F, G, ray, P.steps[ray], w, P.ck + ray * slab, P.seg + lane, P.seg_stride, g);
+    }
+    // block sums in a fixed order
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        for (int off = 32; off > 0; off >>= 1) g[c] += __shfl_xor(g[c], off);
+    if (__lane_id() == 0)
+        for (int c = 0; c < 4; ++c) part[threadIdx.x / 64][c] = g[c];
+    for (int off = 32; off > 0; off >>= 1) mism += __shfl_xor(mism, off);
+    if (__lane_id() == 0 && mism) atomicAdd(P.mismatch, mism);
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double sum = 0.0;
+        for (int wv = 0; wv < (int)(blockDim.x / 64); ++wv) sum += part[wv][threadIdx.x];
+        P.partial[blockIdx.x * 4 + threadIdx.x] = sum;
+    }
+}
+
+}  // namespace
+
+struct bhx_adjoint {
+    bhx_context* ctx = nullptr;
+    bhx_scene scene{};
+    int K = 0;
+    int64_t n_rays = 0;
+    AdjConst host{};
+    DevBuf cst, ck, cls, y, steps, I, order, seg, partial, counters;
+    std::vector<int32_t> order_h;
+    int grid_f = 0, grid_b = 0;
+    double fwd_ms = 0.0, bwd_ms = 0.0;
+    unsigned int mismatches = 0;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+namespace {
+
+int adj_set_scene(bhx_adjoint* A, const bhx_scene* s) {
+    bhx_context* ctx = A->ctx;
+    if (!s) return bhx_fail(ctx, BHX_E_INVALID, "scene is NULL");
+    const int S = std::max(1, s->sqrt_spp);
+    const int64_t n = static_cast<int64_t>(s->width) * s->height * S * S;
+    if (n != A->n_rays) return bhx_fail(ctx, BHX_E_INVALID, "scene size differs from the adjoint's");
+    if (s->max_steps / A->K + 1 != A->host.F.cap && A->host.F.cap != 0)
+        return bhx_fail(ctx, BHX_E_INVALID, "max_steps differs from the adjoint's");
+    AdjConst c;
+    if (!make_frame(*s, A->K, c.F, c.G))
+        return bhx_fail(ctx, BHX_E_INVALID, "only disc scenes without neutron star, jets, Sun or hot spot are differentiable");
+    if (!(s->rtol > 0.0) || !(s->atol > 0.0)) return bhx_fail(ctx, BHX_E_INVALID, "rtol/atol must be > 0");
+    if (!(s->eddington > 0.0)) return bhx_fail(ctx, BHX_E_INVALID, "eddington must be > 0");
+    // the constants are read by kernels of earlier calls: replace them in order
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(A->cst.p, &c, sizeof c, hipMemcpyHostToDevice));
+    A->host = c;
+    A->scene = *s;
+    return BHX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bhx_adjoint_create(bhx_context* ctx, const bhx_scene* s, int32_t K, bhx_adjoint** out) {
+    if (!ctx || !out) return BHX_E_INVALID;
+    *out = nullptr;
+    if (!s) return bhx_fail(ctx, BHX_E_INVALID, "scene is NULL");
+    if (K < 1 || K > 4096) return bhx_fail(ctx, BHX_E_INVALID, "checkpoint interval K must be in [1, 4096]");
+    if (s->width <= 0 || s->height <= 0 || s->max_steps < 1)
+        return bhx_fail(ctx, BHX_E_INVALID, "width/height/max_steps must be > 0");
+    const int S = std::max(1, s->sqrt_spp);
+    const int64_t n = static_cast<int64_t>(s->width) * s->height * S * S;
+    if (n > 2000000000LL) return bhx_fail(ctx, BHX_E_TOO_LARGE, "more than 2e9 samples");
+    const int cap = s->max_steps / K + 1;
+    const double ck_bytes = static_cast<double>(n) * cap * kCk * sizeof(double);
+    if (ck_bytes > 64.0 * (1ull << 30))
+        return bhx_fail(ctx, BHX_E_TOO_LARGE, "checkpoint slab above 64 GiB: raise K or lower max_steps");
+    HIPCHK(hipSetDevice(ctx->device));
+    bhx_adjoint* A = new bhx_adjoint();
+    A->ctx = ctx;
+    A->K = K;
+    A->n_rays = n;
+    auto bail = [&](int rc) {
+        bhx_adjoint_destroy(A);
+        return rc;
+    };
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adj_forward, 256, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    A->grid_f = per_cu * ctx->n_cu;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adj_backward, 256, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    A->grid_b = per_cu * ctx->n_cu;
+    const int64_t lanes = static_cast<int64_t>(A->grid_b) * 256;
+    if (A->cst.ensure(sizeof(AdjConst)) != hipSuccess || A->ck.ensure(static_cast<size_t>(ck_bytes)) != hipSuccess ||
+        A->cls.ensure(n) != hipSuccess || A->y.ensure(sizeof(double) * 8 * n) != hipSuccess ||
+        A->steps.ensure(sizeof(int32_t) * n) != hipSuccess || A->I.ensure(sizeof(double) * n) != hipSuccess ||
+        A->order.ensure(sizeof(int32_t) * n) != hipSuccess ||
+        A->seg.ensure(sizeof(double) * 5 * K * lanes) != hipSuccess ||
+        A->partial.ensure(sizeof(double) * 4 * A->grid_b) != hipSuccess ||
+        A->counters.ensure(64) != hipSuccess)
+        return bail(bhx_fail(ctx, BHX_E_NOMEM, "adjoint buffers"));
+    for (auto& e : A->ev)
+        if (hipEventCreate(&e) != hipSuccess) return bail(bhx_fail(ctx, BHX_E_HIP, "hipEventCreate"));
+    const int rc = adj_set_scene(A, s);
+    if (rc) return bail(rc);
+    *out = A;
+    return BHX_OK;
+}
+
+int bhx_adjoint_destroy(bhx_adjoint* A) {
+    if (!A) return BHX_OK;
+    (void)hipSetDevice(A->ctx->device);
+    (void)hipDeviceSynchronize();
+    for (DevBuf* d : {&A->cst, &A->ck, &A->cls, &A->y, &A->steps, &A->I, &A->order, &A->seg, &A->partial,
+                      &A->counters})
+        d->release();
+    for (auto& e : A->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete A;
+    return BHX_OK;
+}
+
+int bhx_adjoint_set_scene(bhx_adjoint* A, const bhx_scene* s) {
+    if (!A) return BHX_E_INVALID;
+    bhx_context* ctx = A->ctx;
+    HIPCHK(hipSetDevice(ctx->device));
+    return adj_set_scene(A, s);
+}
+
+int bhx_adjoint_forward(bhx_adjoint* A, double* I, void* stream) {
+    if (!A) return BHX_E_INVALID;
+    bhx_context* ctx = A->ctx;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    AdjArgs P{};
+    P.n_rays = A->n_rays;
+    P.counter = static_cast<uint32_t*>(A->counters.p);
+    P.ck = static_cast<double*>(A->ck.p);
+    P.cls = static_cast<uint8_t*>(A->cls.p);
+    P.y = static_cast<double*>(A->y.p);
+    P.steps = static_cast<int32_t*>(A->steps.p);
+    P.I = I ? I : static_cast<double*>(A->I.p);
+    HIPCHK(launch_zero_u64(static_cast<unsigned long long*>(A->counters.p), 8, st));
+    HIPCHK(hipEventRecord(A->ev[0], st));
+    hipLaunchKernelGGL(k_adj_forward, dim3(A->grid_f), dim3(256), 0, st,
+                       static_cast<const AdjConst*>(A->cst.p), P);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(A->ev[1], st));
+    // the backward's ray order: disc hits, most accepted steps first
+    std::vector<uint8_t> cls(A->n_rays);
+    std::vector<int32_t> steps(A->n_rays);
+    HIPCHK(hipMemcpyAsync(cls.data(), A->cls.p, A->n_rays, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(steps.data(), A->steps.p, sizeof(int32_t) * A->n_rays, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    A->order_h.clear();
+    for (int64_t r = 0; r < A->n_rays; ++r)
+        if (cls[r] == BHX_DISC) A->order_h.push_back(static_cast<int32_t>(r));
+    std::stable_sort(A->order_h.begin(), A->order_h.end(),
+                     [&](int32_t a, int32_t b) { return steps[a] > steps[b]; });
+    if (!A->order_h.empty())
+        HIPCHK(hipMemcpy(A->order.p, A->order_h.data(), sizeof(int32_t) * A->order_h.size(),
+                         hipMemcpyHostToDevice));
+    A->fwd_ms = 0.0;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, A->ev[0], A->ev[1]) == hipSuccess) A->fwd_ms = ms;
+    return BHX_OK;
+}
+
+int bhx_adjoint_backward(bhx_adjoint* A, const double* w, double* grad, void* stream) {
+    if (!A || !w || !grad) return BHX_E_INVALID;
+    bhx_context* ctx = A->ctx;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    AdjArgs P{};
+    P.n_rays = A->n_rays;
+    P.ck = static_cast<double*>(A->ck.p);
+    P.steps = static_cast<int32_t*>(A->steps.p);
+    P.w = w;
+    P.order = static_cast<const int32_t*>(A->order.p);
+    P.n_order = static_cast<int32_t>(A->order_h.size());
+    P.seg = static_cast<double*>(A->seg.p);
+    P.seg_stride = static_cast<int64_t>(A->grid_b) * 256;
+    P.partial = static_cast<double*>(A->partial.p);
+    P.mismatch = reinterpret_cast<unsigned int*>(static_cast<char*>(A->counters.p) + 32);
+    HIPCHK(launch_zero_u64(static_cast<unsigned long long*>(A->counters.p), 8, st));
+    HIPCHK(hipEventRecord(A->ev[2], st));
+    hipLaunchKernelGGL(k_adj_backward, dim3(A->grid_b), dim3(256), 0, st,
+                       static_cast<const AdjConst*>(A->cst.p), P);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(A->ev[3], st));
+    std::vector<double> part(4 * static_cast<size_t>(A->grid_b));
+    unsigned int mism = 0;
+    HIPCHK(hipMemcpyAsync(part.data(), A->partial.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&mism, P.mismatch, sizeof mism, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int c = 0; c < 4; ++c) grad[c] = 0.0;
+    for (int b = 0; b < A->grid_b; ++b)
+        for (int c = 0; c < 4; ++c) grad[c] += part[4 * b + c];
+    A->mismatches = mism;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, A->ev[2], A->ev[3]) == hipSuccess) A->bwd_ms = ms;
+    return BHX_OK;
+}
+
+int bhx_adjoint_rays(const bhx_adjoint* A, bhx_rays* rays) {
+    if (!A || !rays) return BHX_E_INVALID;
+    std::memset(rays, 0, sizeof *rays);
+    rays->cls = static_cast<uint8_t*>(A->cls.p);
+    rays->y = static_cast<double*>(A->y.p);
+    rays->steps = static_cast<int32_t*>(A->steps.p);
+    return BHX_OK;
+}
+
+int bhx_adjoint_copy_rays(bhx_adjoint* A, uint8_t* cls, double* y, int32_t* steps, void* stream) {
+    if (!A) return BHX_E_INVALID;
+    bhx_context* ctx = A->ctx;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    if (cls) HIPCHK(hipMemcpy(cls, A->cls.p, A->n_rays, hipMemcpyDeviceToHost));
+    if (y) HIPCHK(hipMemcpy(y, A->y.p, sizeof(double) * 8 * A->n_rays, hipMemcpyDeviceToHost));
+    if (steps) HIPCHK(hipMemcpy(steps, A->steps.p, sizeof(int32_t) * A->n_rays, hipMemcpyDeviceToHost));
+    return BHX_OK;
+}
+
+int bhx_adjoint_info(const bhx_adjoint* A, int64_t* ck_bytes, int64_t* disc_rays, int64_t* mismatches,
+                     double* fwd_ms, double* bwd_ms) {
+    if (!A) return BHX_E_INVALID;
+    if (ck_bytes) *ck_bytes = static_cast<int64_t>(A->ck.bytes);
+    if (disc_rays) *disc_rays = static_cast<int64_t>(A->order_h.size());
+    if (mismatches) *mismatches = A->mismatches;
+    if (fwd_ms) *fwd_ms = A->fwd_ms;
+    if (bwd_ms) *bwd_ms = A->bwd_ms;
+    return BHX_OK;
+}
+
+}  // extern "C"

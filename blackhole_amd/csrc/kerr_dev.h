@@ -22,12 +22,16 @@ namespace bhx {
 // 1/x to <= 1 ulp: hardware estimate + two Newton steps.  Non-finite or zero
 // x gives a non-finite result, which the step controller rejects like the
 // reference's inf/NaN from a division by zero.
-__device__ __forceinline__ double rcp(double x) {
+__host__ __device__ __forceinline__ double rcp(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
     double r = __builtin_amdgcn_rcp(x);
     double e = fma(-x, r, 1.0);
     r = fma(r, e, r);
     e = fma(-x, r, 1.0);
     return fma(r, e, r);
+#else
+    return 1.0 / x;   // host builds of these functions (adjoint self-checks)
+#endif
 }
 
 // sin and cos of |x| <= 1e5 to <= 1 ulp (checked against glibc): Cody-Waite
@@ -37,7 +41,7 @@ __device__ __forceinline__ double rcp(double x) {
 // for the library routine whose Payne-Hanek branch would split every RHS into
 // separate basic blocks.  theta cannot leave that range: each DP5 step moves it
 // by less than pi and a ray takes at most 20000 steps.  NaN stays NaN.
-__device__ __forceinline__ void sincos_near(double x, double* s, double* c) {
+__host__ __device__ __forceinline__ void sincos_near(double x, double* s, double* c) {
     const double k = rint(x * 0.63661977236758134308);
     double r = fma(-k, 1.5707963267948966, x);
     r = fma(-k, 6.123233995736766e-17, r);
@@ -62,7 +66,7 @@ __device__ __forceinline__ void sincos_near(double x, double* s, double* c) {
 // exp(x) for |x| < 700 to <= 1 ulp (checked against glibc): Cody-Waite
 // reduction by ln 2 (fdlibm's two-part constant), degree-13 Taylor-Horner on
 // |r| <= ln(2)/2 (remainder < 4e-18), scale by 2^k.
-__device__ __forceinline__ double exp_near(double x) {
+__host__ __device__ __forceinline__ double exp_near(double x) {
     const double k = rint(x * 1.4426950408889634);
     double r = fma(-k, 6.93147180369123816490e-01, x);
     r = fma(-k, 1.90821492927058770002e-10, r);
@@ -85,7 +89,7 @@ __device__ __forceinline__ double exp_near(double x) {
 
 // log(x) for finite normal x > 0 to <= 1 ulp (checked against glibc): the
 // reduction and minimax kernel of fdlibm's __ieee754_log (Sun, 1993).
-__device__ __forceinline__ double log_pos(double x) {
+__host__ __device__ __forceinline__ double log_pos(double x) {
     int e;
     double m = frexp(x, &e);
     if (m < 0.70710678118654752440) { m *= 2.0; e -= 1; }
@@ -101,9 +105,12 @@ __device__ __forceinline__ double log_pos(double x) {
     return dk * 6.93147180369123816490e-01 - ((hfsq - (s * (hfsq + R) + dk * 1.90821492927058770002e-10)) - f);
 }
 
-__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }  // std::max
-__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }  // std::min
-__device__ __forceinline__ double dclamp(double v, double lo, double hi) {             // std::clamp
+// true for finite x; the compiler builtin, so host and device builds agree
+__host__ __device__ __forceinline__ bool finite_d(double x) { return __builtin_isfinite(x); }
+
+__host__ __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }  // std::max
+__host__ __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }  // std::min
+__host__ __device__ __forceinline__ double dclamp(double v, double lo, double hi) {             // std::clamp
     return (v < lo) ? lo : ((hi < v) ? hi : v);
 }
 
@@ -112,7 +119,7 @@ struct RayK {
     double a, a2, pt, pph;
     double apt, apph, a2pt2, pph2;   // a p_t, a p_phi, a^2 p_t^2, p_phi^2
 };
-__device__ __forceinline__ RayK ray_k(double a, double a2, double pt, double pph) {
+__host__ __device__ __forceinline__ RayK ray_k(double a, double a2, double pt, double pph) {
     RayK k;
     k.a = a; k.a2 = a2; k.pt = pt; k.pph = pph;
     k.apt = a * pt; k.apph = a * pph; k.a2pt2 = a2 * pt * pt; k.pph2 = pph * pph;
@@ -137,7 +144,7 @@ struct Rhs {
 // with H = g^{ab} p_a p_b.  Algebraically identical to the component-wise
 // partials of the reference (checked against the oracle's RHS to rounding),
 // about 40% fewer operations, and a single reciprocal of Sigma Delta sin^2.
-__device__ __forceinline__ void kerr_rhs(const RayK& K, double r, double th, double pr, double pth,
+__host__ __device__ __forceinline__ void kerr_rhs(const RayK& K, double r, double th, double pr, double pth,
                                          Rhs& o) {
     double sn, cs;
     sincos_near(th, &sn, &cs);
@@ -178,13 +185,13 @@ __device__ __forceinline__ void kerr_rhs(const RayK& K, double r, double th, dou
 
 // splitmix64 (the reference's Rng, src/scene.h:27-44), counter form: draw n
 // (1-based) of a stream seeded with s is mix(s + n * golden).
-__device__ __forceinline__ uint64_t splitmix_draw(uint64_t seed, uint64_t n) {
+__host__ __device__ __forceinline__ uint64_t splitmix_draw(uint64_t seed, uint64_t n) {
     uint64_t z = seed + n * 0x9E3779B97F4A7C15ull;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
 }
-__device__ __forceinline__ double splitmix_u01(uint64_t seed, uint64_t n) {
+__host__ __device__ __forceinline__ double splitmix_u01(uint64_t seed, uint64_t n) {
     return (double)(splitmix_draw(seed, n) >> 11) * 0x1.0p-53;
 }
 

@@ -21,6 +21,9 @@
 //             by ballot + prefix rank (mbcnt) - consecutive slots, so the
 //             refill loads are coalesced; one loop iteration = one DP5 attempt
 //             on every live lane.
+#include "dense_dev.h"
+#include "dp5.h"
+#include "raygen_dev.h"
 #include "events_dev.h"
 #include "kerr_dev.h"
 #include "params.h"
@@ -31,26 +34,7 @@
 namespace bhx {
 namespace {
 
-// Dormand-Prince 5(4) tableau, error weights and Hairer's dense-output
-// weights (Hairer, Norsett & Wanner, Solving ODEs I, II.4).
-constexpr double A21 = 1.0 / 5.0;
-constexpr double A31 = 3.0 / 40.0, A32 = 9.0 / 40.0;
-constexpr double A41 = 44.0 / 45.0, A42 = -56.0 / 15.0, A43 = 32.0 / 9.0;
-constexpr double A51 = 19372.0 / 6561.0, A52 = -25360.0 / 2187.0, A53 = 64448.0 / 6561.0,
-                 A54 = -212.0 / 729.0;
-constexpr double A61 = 9017.0 / 3168.0, A62 = -355.0 / 33.0, A63 = 46732.0 / 5247.0,
-                 A64 = 49.0 / 176.0, A65 = -5103.0 / 18656.0;
-constexpr double B1 = 35.0 / 384.0, B3 = 500.0 / 1113.0, B4 = 125.0 / 192.0,
-                 B5 = -2187.0 / 6784.0, B6 = 11.0 / 84.0;
-constexpr double E1 = B1 - 5179.0 / 57600.0, E3 = B3 - 7571.0 / 16695.0,
-                 E4 = B4 - 393.0 / 640.0, E5 = B5 - (-92097.0 / 339200.0),
-                 E6 = B6 - 187.0 / 2100.0, E7 = 0.0 - 1.0 / 40.0;
-constexpr double D1 = -12715105075.0 / 11282082432.0, D3 = 87487479700.0 / 32700410799.0,
-                 D4 = -10690763975.0 / 1880347072.0, D5 = 701980252875.0 / 199316789632.0,
-                 D6 = -1453857185.0 / 822651844.0, D7 = 69997945.0 / 29380423.0;
-// PI controller exponents alpha = 0.7/5, beta = 0.4/5, safety 0.9 (geodesic.h:170).
-constexpr double kAlpha = 0.7 / 5.0, kBeta = 0.4 / 5.0;
-constexpr double kLog1em4 = -9.210340371976182;  // log(1e-4): err_prev floor
+using namespace dp5;
 
 constexpr int kTile = 8;
 
@@ -59,88 +43,6 @@ __constant__ const double kGLx[4] = {0.06943184420297371, 0.33000947820757187, 0
                                      0.93056815579702629};
 __constant__ const double kGLw[4] = {0.17392742256872693, 0.32607257743127307, 0.32607257743127307,
                                      0.17392742256872693};
-
-__device__ __forceinline__ double dense_comp(double c1, double c2, double c3, double c4, double c5,
-                                             double s) {
-    const double s1 = 1.0 - s;
-    return c1 + s * (c2 + s1 * (c3 + s * (c4 + s1 * c5)));
-}
-
-// cos(th) * c0 <= 0 for c0 = cos of a double (never 0): the sign of cos(th)
-// is read off th near the equator, where cos(fl(pi/2)) = 6.1e-17 > 0 and the
-// next double up has a negative cosine; elsewhere cos is evaluated.
-__device__ __forceinline__ bool crossed(double th, double c0) {
-    if (th > 0.5 && th < 2.6) {
-        const bool cpos = th <= 1.5707963267948966;
-        return (c0 > 0.0) ? !cpos : cpos;
-    }
-    double s, c;
-    sincos_near(th, &s, &c);
-    return c * c0 <= 0.0;
-}
-
-// Value and s-derivative of one component of the dense-output polynomial.
-__device__ __forceinline__ void dense_vd(double c1, double c2, double c3, double c4, double c5,
-                                         double s, double& v, double& dv) {
-    const double s1 = 1.0 - s;
-    const double h4 = c4 + s1 * c5;
-    const double g = c3 + s * h4;
-    const double dg = h4 - s * c5;
-    const double m = c2 + s1 * g;
-    const double dm = s1 * dg - g;
-    v = c1 + s * m;
-    dv = m + s * dm;
-}
-
-// Event roots on the dense output.  The reference bisects 60 times on the
-// predicate (cos(theta(s)) c0 <= 0 for the disc, r(s) <= R for the star) and
-// returns the midpoint of the last bracket (render.cpp:160-164, 184-190).
-// Here the same predicate keeps a bracket while Newton's method on the
-// smooth function behind it (theta(s) - fl(pi/2), r(s) - R) converges
-// quadratically from the secant estimate: the same root to rounding in ~4
-// iterations instead of 60, so one lane's event no longer stalls its wave.
-// Bisection steps are taken whenever Newton leaves the bracket.
-template <typename Pred>
-__device__ __forceinline__ double newton_root(double c1, double c2, double c3, double c4, double c5,
-                                              double target, double f0, double f1, Pred hit) {
-    double lo = 0.0, hi = 1.0;
-    double s = (f0 != f1) ? f0 / (f0 - f1) : 0.5;
-    if (!(s > 0.0 && s < 1.0)) s = 0.5;
-    for (int it = 0; it < 12; ++it) {
-        double v, dv;
-        dense_vd(c1, c2, c3, c4, c5, s, v, dv);
-        if (hit(v)) hi = s; else lo = s;
-        double sn = s - (v - target) / dv;
-        if (sn == s) break;   // on the root to rounding (e.g. v == target exactly)
-        if (!(sn >= lo && sn <= hi)) sn = 0.5 * (lo + hi);
-        const bool done = fabs(sn - s) <= 4e-16;
-        s = sn;
-        if (done) break;
-    }
-    return s;
-}
-
-// theta(s) crosses the equatorial plane; c0 = cos(theta(0)).
-__device__ __forceinline__ double root_theta(double c1, double c2, double c3, double c4, double c5,
-                                             double c0, double th0, double th1) {
-    if (th0 > 0.5 && th0 < 2.6 && th1 > 0.5 && th1 < 2.6) {
-        const double T = 1.5707963267948966;
-        return newton_root(c1, c2, c3, c4, c5, T, th0 - T, th1 - T,
-                           [c0](double th) { return crossed(th, c0); });
-    }
-    double lo = 0.0, hi = 1.0;   // crossing through a pole's image: plain bisection
-    for (int it = 0; it < 60; ++it) {
-        const double mid = 0.5 * (lo + hi);
-        if (crossed(dense_comp(c1, c2, c3, c4, c5, mid), c0)) hi = mid; else lo = mid;
-    }
-    return 0.5 * (lo + hi);
-}
-
-// r(s) reaches the neutron-star surface R.
-__device__ __forceinline__ double root_r(double c1, double c2, double c3, double c4, double c5,
-                                         double R, double r0, double r1) {
-    return newton_root(c1, c2, c3, c4, c5, R, r0 - R, r1 - R, [R](double r) { return r <= R; });
-}
 
 // ---------------------------------------------------------------------------
 // Ray generation: one block of 64 threads per queue tile.
@@ -161,38 +63,10 @@ __global__ void __launch_bounds__(64) k_raygen(const RaygenArgs P) {
         const int px = x0 + pin % tw, py = y0 + pin / tw;
         const int q = base + i;
         const int ray = P.ray_base + (py * P.W + px) * S2 + k;
-        // jitter (render.cpp:440-446): draws 2k+1, 2k+2 of the pixel's stream
-        const uint64_t sd = P.seed ^ ((uint64_t)py << 32) ^ ((uint64_t)px * 0x9E3779B97F4A7C15ull);
-        const int sx = k % P.S, sy = k / P.S;
-        const double jx = (sx + splitmix_u01(sd, 2 * (uint64_t)k + 1)) * P.inv_S;
-        const double jy = (sy + splitmix_u01(sd, 2 * (uint64_t)k + 2)) * P.inv_S;
-        const double u = (2.0 * (px + jx) / P.W - 1.0);
-        const double v = (1.0 - 2.0 * (py + jy) / P.H);
-        // Camera::ray_direction (scene.h:356-384)
-        double cx = u * P.tanf, cy = v * P.tanf / P.aspect;
-        const double cz = 1.0;
-        if (P.has_roll) {
-            const double xr = P.c_roll * cx - P.s_roll * cy;
-            const double yr = P.s_roll * cx + P.c_roll * cy;
-            cx = xr;
-            cy = yr;
-        }
-        const double y2 = P.c_pitch * cy - P.s_pitch * cz;
-        const double z2 = P.s_pitch * cy + P.c_pitch * cz;
-        const double x2 = P.c_yaw * cx + P.s_yaw * z2;
-        const double z3 = -P.s_yaw * cx + P.c_yaw * z2;
-        const double v0 = -z3, v1 = -y2, v2 = x2;
-        const double n = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
-        const double d0 = v0 / n, d1 = v1 / n, d2 = v2 / n;
-        // make_photon, past-directed (geodesic.h:215-231): p^mu = -(e0 - d.e), lowered
-        const double pu0 = -P.e00;
-        const double pu1 = d0 * P.e11;
-        const double pu2 = d1 * P.e22;
-        const double pu3 = -(P.e03 - d2 * P.e33);
-        const double pt = P.gl_tt * pu0 + P.gl_tp * pu3;
-        const double pr = P.gl_rr * pu1;
-        const double pth = P.gl_hh * pu2;
-        const double pph = P.gl_tp * pu0 + P.gl_pp * pu3;
+        double d[3];
+        camera_dir(P, px, py, k, d);
+        double pt, pr, pth, pph;
+        camera_photon(P, d, pt, pr, pth, pph);
         // Carter Q at the camera (kerr.h:342-348)
         const double E = -pt;
         const double q0 = pth * pth + P.cam_cs2 * (P.a2 * (0.0 - E * E) + pph * pph / P.cam_sn2);

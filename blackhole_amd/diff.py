@@ -1,0 +1,154 @@
+"""Differentiable disc images: the bhx_adjoint C ABI under torch.autograd.
+
+    dr = DiscRender(tracer, scene, K=64)
+    p = torch.tensor([spin, cam_theta, eddington, cam_r], dtype=torch.float64,
+                     requires_grad=True)
+    I = dr(p)                               # (H, W) bolometric intensity, W m^-2 sr^-1
+    loss = ((blur(I) - blur(I_target)) ** 2).sum()
+    loss.backward()                         # p.grad from the discrete adjoint
+
+The forward is the reference's disc image (src/disc.h:94-121,
+src/render.cpp:280-300: I = sigma (g T)^4 / pi on disc hits, 0 elsewhere) with
+the camera fixed in everything but (spin, cam_theta, eddington, cam_r); the
+backward runs the hand-written adjoint kernel (blackhole_amd/csrc/adjoint.hip).
+The gradient is that of the smooth part of the image: pixels whose class
+changes under an infinitesimal parameter change (disc edges, the shadow's
+rim) contribute boundary terms that no pointwise derivative sees.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import native
+from .abi import Scene
+
+PARAMS = ("spin", "cam_theta", "eddington", "cam_r")
+
+
+def with_params(scene, p):
+    """Copy of scene with (spin, cam_theta, eddington, cam_r) = p; r_escape
+    and the rest of the camera stay as they are."""
+    s = Scene()
+    C.pointer(s)[0] = scene
+    s.spin, s.cam_theta, s.eddington, s.cam_r = (float(v) for v in p)
+    return s
+
+
+class DiscRender:
+    """One bhx_adjoint (checkpoint slab, scratch) for frames of one size."""
+
+    def __init__(self, tracer, scene, K=64):
+        self.tr = tracer
+        self.lib = tracer.lib
+        self.scene = scene
+        self.K = int(K)
+        self.adj = C.c_void_p()
+        native.check(tracer.ctx, self.lib.bhx_adjoint_create(tracer.ctx, C.byref(scene), self.K,
+                                                             C.byref(self.adj)), "bhx_adjoint_create")
+
+    def close(self):
+        if self.adj:
+            self.lib.bhx_adjoint_destroy(self.adj)
+            self.adj = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- raw entry points --------------------------------------------------
+    def forward_raw(self, p, out=None):
+        """I per sample (device tensor) at parameters p; records checkpoints."""
+        t = self.tr.torch
+        s = with_params(self.scene, p)
+        native.check(self.tr.ctx, self.lib.bhx_adjoint_set_scene(self.adj, C.byref(s)), "bhx_adjoint_set_scene")
+        if out is None:
+            out = t.empty(s.n_rays, dtype=t.float64, device=f"cuda:{self.tr.device}")
+        native.check(self.tr.ctx, self.lib.bhx_adjoint_forward(self.adj, C.c_void_p(out.data_ptr()),
+                                                               self.tr._stream(None)), "bhx_adjoint_forward")
+        return out
+
+    def backward_raw(self, w):
+        """dL/d(spin, cam_theta, eddington, cam_r) for w = dL/dI per sample."""
+        w = w.contiguous()
+        g = (C.c_double * 4)()
+        native.check(self.tr.ctx, self.lib.bhx_adjoint_backward(self.adj, C.c_void_p(w.data_ptr()), g,
+                                                                self.tr._stream(None)), "bhx_adjoint_backward")
+        return np.array(g[:])
+
+    def rays(self):
+        """Class, hit state (8, n) and accepted steps of the last forward (host copies)."""
+        n = self.scene.n_rays
+        out = {"cls": np.zeros(n, np.uint8), "y": np.zeros((8, n)), "steps": np.zeros(n, np.int32)}
+        native.check(self.tr.ctx, self.lib.bhx_adjoint_copy_rays(
+            self.adj, *[out[k].ctypes.data_as(C.c_void_p) for k in ("cls", "y", "steps")],
+            self.tr._stream(None)), "bhx_adjoint_copy_rays")
+        return out
+
+    def info(self):
+        ck, disc, mism = C.c_int64(), C.c_int64(), C.c_int64()
+        f, b = C.c_double(), C.c_double()
+        native.check(self.tr.ctx, self.lib.bhx_adjoint_info(self.adj, C.byref(ck), C.byref(disc), C.byref(mism),
+                                                            C.byref(f), C.byref(b)), "bhx_adjoint_info")
+        return {"ck_bytes": ck.value, "disc_rays": disc.value, "mismatches": mism.value,
+                "fwd_ms": f.value, "bwd_ms": b.value}
+
+    # ---- autograd ------------------------------------------------------------
+    def __call__(self, p):
+        return _DiscImage.apply(p, self)
+
+
+def _image_from_samples(tr, I, scene):
+    S2 = max(1, scene.sqrt_spp) ** 2
+    return I.view(scene.height, scene.width, S2).mean(-1)
+
+
+import torch  # noqa: E402  (torch is the autograd host; the arithmetic is in libbhx.so)
+
+
+class _DiscImage(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, p, dr):
+        I = dr.forward_raw(p.detach().cpu().numpy())
+        ctx.dr = dr
+        ctx.device = p.device
+        return _image_from_samples(dr.tr, I, dr.scene)
+
+    @staticmethod
+    def backward(ctx, gI):
+        dr = ctx.dr
+        sc = dr.scene
+        S2 = max(1, sc.sqrt_spp) ** 2
+        w = (gI.to(torch.float64).unsqueeze(-1).expand(sc.height, sc.width, S2) / S2).contiguous().view(-1)
+        g = dr.backward_raw(w)
+        return torch.tensor(g, dtype=torch.float64, device=ctx.device), None
+
+
+def gaussian_blur(img, sigma=2.0):
+    """Separable Gaussian of radius ceil(3 sigma), clamp-to-edge, normalised
+    weights - the blur of img::add_glare (src/image.cpp:61-93) - on a 2-D
+    tensor, differentiable."""
+    r = max(1, int(math.ceil(3.0 * sigma)))
+    i = torch.arange(-r, r + 1, device=img.device)
+    k = torch.exp(-0.5 * (i.to(torch.float64) ** 2) / (sigma * sigma))
+    k = k / k.sum()
+    H, W = img.shape
+    ix = (torch.arange(W, device=img.device)[:, None] + i[None, :]).clamp(0, W - 1)
+    tmp = (img[:, ix] * k).sum(-1)
+    iy = (torch.arange(H, device=img.device)[:, None] + i[None, :]).clamp(0, H - 1)
+    return (tmp[iy, :] * k[None, :, None]).sum(1)
+
+
+def gaussian_blur_np(img, sigma=2.0):
+    """gaussian_blur on a numpy array (the same sums, for CPU checks)."""
+    r = max(1, int(math.ceil(3.0 * sigma)))
+    i = np.arange(-r, r + 1)
+    k = np.exp(-0.5 * (i.astype(np.float64) ** 2) / (sigma * sigma))
+    k = k / k.sum()
+    H, W = img.shape
+    ix = np.clip(np.arange(W)[:, None] + i[None, :], 0, W - 1)
+    tmp = (img[:, ix] * k).sum(-1)
+    iy = np.clip(np.arange(H)[:, None] + i[None, :], 0, H - 1)
+    return (tmp[iy, :] * k[None, :, None]).sum(1)

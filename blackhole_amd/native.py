@@ -19,7 +19,9 @@ SYMBOLS = (
     "bhx_blackbody_xyz_dev", "bhx_set_launch", "bhx_get_launch", "bhx_setup_jet",
     "bhx_polytrope_star", "bhx_setup_neutron_star", "bhx_batch_create", "bhx_batch_destroy",
     "bhx_batch_run", "bhx_batch_scene", "bhx_batch_copy_scene", "bhx_batch_stats",
-    "bhx_batch_set_launch", "bhx_batch_get_launch", "bhx_batch_autotune",
+    "bhx_batch_set_launch", "bhx_batch_get_launch", "bhx_batch_autotune", "bhx_adjoint_create",
+    "bhx_adjoint_destroy", "bhx_adjoint_set_scene", "bhx_adjoint_forward", "bhx_adjoint_backward",
+    "bhx_adjoint_rays", "bhx_adjoint_copy_rays", "bhx_adjoint_info",
 )
 
 
@@ -65,6 +67,15 @@ def lib():
     L.bhx_batch_set_launch.argtypes = [P, C.POINTER(Launch)]
     L.bhx_batch_get_launch.argtypes = [P, C.POINTER(Launch)]
     L.bhx_batch_autotune.argtypes = [P, C.c_int32, C.POINTER(Launch), C.POINTER(C.c_double), P]
+    L.bhx_adjoint_create.argtypes = [P, C.POINTER(Scene), C.c_int32, C.POINTER(P)]
+    L.bhx_adjoint_destroy.argtypes = [P]
+    L.bhx_adjoint_set_scene.argtypes = [P, C.POINTER(Scene)]
+    L.bhx_adjoint_forward.argtypes = [P, P, P]
+    L.bhx_adjoint_backward.argtypes = [P, P, P, P]
+    L.bhx_adjoint_rays.argtypes = [P, C.POINTER(Rays)]
+    L.bhx_adjoint_copy_rays.argtypes = [P, P, P, P, P]
+    L.bhx_adjoint_info.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]
     _lib = L
     return L
 

@@ -166,3 +166,14 @@ def survey_batch(seed=20240921, n=64, n_ns=8, side_min=128, side_max=1024):
             s.cam_fov = 5.0 * math.atan2(b_app, s.cam_r)
         out.append(s)
     return out
+
+
+def disc_frame(width=512, height=512, spin=0.7, incl_deg=45.0, eddington=0.1, r_cam=500.0):
+    """The differentiation scene: NT disc ISCO-20M, no stars, camera at r_cam
+    (default 500 M) and inclination incl_deg over a +-20 M image plane, CLI
+    tolerances; r_escape fixed at 3 * 500 + 100 so that varying r_cam does not
+    move the escape sphere."""
+    s = plain_scene(spin=spin, incl_deg=incl_deg, r_cam=r_cam, width=width, height=height,
+                    fov=2.0 * math.atan(20.0 / 500.0), disc_r_out=20.0, eddington=eddington)
+    s.r_escape = 3.0 * 500.0 + 100.0
+    return s

@@ -1,0 +1,254 @@
+// selfcheck.cpp - the adjoint's hand-written reverse mode against central
+// finite differences of the same forward code, on the CPU (host builds of the
+// device functions).  Built and run by tests/test_adjoint_selfcheck.py:
+//   rhs    vector-Jacobian product of the Hamiltonian right-hand side
+//   step   one Dormand-Prince step with the step size frozen
+//   term   the terminal step: r at the equatorial crossing of the dense output
+//   ray    d I / d(spin, theta_cam, eddington, r_cam) of whole disc-hit rays
+//          against finite differences of the full adaptive forward
+// Prints one line per check: name, count, worst relative error.
+#include "../../blackhole_amd/csrc/adjoint_frame.h"
+
+#include <cstdio>
+#include <random>
+#include <vector>
+
+using namespace bhx;
+using namespace bhx::adj;
+
+static double rel(double a, double b, double scale) { return std::fabs(a - b) / std::max(std::fabs(b), scale); }
+
+static void check_rhs(std::mt19937_64& rng) {
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    double worst = 0.0;
+    int n = 0;
+    for (int t = 0; t < 200; ++t) {
+        const double a = 0.998 * U(rng), pt = -1.0, pph = -5.0 + 10.0 * U(rng);
+        const double x[4] = {3.0 + 40.0 * U(rng), 0.3 + 2.5 * U(rng), -2.0 + 4.0 * U(rng), -6.0 + 12.0 * U(rng)};
+        const double w[4] = {U(rng) - 0.5, U(rng) - 0.5, U(rng) - 0.5, U(rng) - 0.5};
+        double xb[4] = {}, cb[3] = {};
+        rhs_vjp(ray_k(a, a * a, pt, pph), x, w, xb, cb);
+        auto J = [&](const double* xx, double aa, double ptt, double pphh) {
+            double k[4];
+            rhs_x(ray_k(aa, aa * aa, ptt, pphh), xx, k);
+            return w[0] * k[0] + w[1] * k[1] + w[2] * k[2] + w[3] * k[3];
+        };
+        double fd[7];
+        for (int i = 0; i < 4; ++i) {
+            double xp[4] = {x[0], x[1], x[2], x[3]}, xm[4] = {x[0], x[1], x[2], x[3]};
+            const double h = 1e-6 * std::max(1.0, std::fabs(x[i]));
+            xp[i] += h; xm[i] -= h;
+            fd[i] = (J(xp, a, pt, pph) - J(xm, a, pt, pph)) / (2 * h);
+        }
+        const double h = 1e-6;
+        fd[4] = (J(x, a + h, pt, pph) - J(x, a - h, pt, pph)) / (2 * h);
+        fd[5] = (J(x, a, pt + h, pph) - J(x, a, pt - h, pph)) / (2 * h);
+        fd[6] = (J(x, a, pt, pph + h) - J(x, a, pt, pph - h)) / (2 * h);
+        const double ad[7] = {xb[0], xb[1], xb[2], xb[3], cb[0], cb[1], cb[2]};
+        double scale = 0.0;
+        for (int i = 0; i < 7; ++i) scale = std::max(scale, std::fabs(fd[i]));
+        for (int i = 0; i < 7; ++i) worst = std::max(worst, rel(ad[i], fd[i], 1e-3 * scale));
+        ++n;
+    }
+    std::printf("rhs %d %.3e\n", n, worst);
+}
+
+static void step_fwd(const RayK& K, const double x[4], double h, double yn[4]) {
+    double ynb[4] = {0, 0, 0, 0}, kb[7][4] = {}, xb[4] = {}, cb[3] = {};
+    (void)ynb; (void)kb; (void)xb; (void)cb;
+    const double A[6][5] = {{0, 0, 0, 0, 0}, {A21, 0, 0, 0, 0}, {A31, A32, 0, 0, 0}, {A41, A42, A43, 0, 0},
+                            {A51, A52, A53, A54, 0}, {A61, A62, A63, A64, A65}};
+    const double B[6] = {B1, 0.0, B3, B4, B5, B6};
+    double k[6][4], X[4];
+    for (int i = 0; i < 6; ++i) {
+        for (int c = 0; c < 4; ++c) {
+            double acc = 0.0;
+            for (int j = 0; j < i; ++j) acc += A[i][j] * k[j][c];
+            X[c] = x[c] + h * acc;
+        }
+        rhs_x(K, X, k[i]);
+    }
+    for (int c = 0; c < 4; ++c) {
+        double acc = 0.0;
+        for (int j = 0; j < 6; ++j) acc += B[j] * k[j][c];
+        yn[c] = x[c] + h * acc;
+    }
+}
+
+static void check_step(std::mt19937_64& rng) {
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    double worst = 0.0;
+    int n = 0;
+    for (int t = 0; t < 100; ++t) {
+        const double a = 0.998 * U(rng), pt = -1.0, pph = -5.0 + 10.0 * U(rng);
+        const double x[4] = {4.0 + 40.0 * U(rng), 0.5 + 2.0 * U(rng), -1.0 + 2.0 * U(rng), -3.0 + 6.0 * U(rng)};
+        const double h = -(0.05 + 0.5 * U(rng));
+        const double w[4] = {U(rng) - 0.5, U(rng) - 0.5, U(rng) - 0.5, U(rng) - 0.5};
+        double ynb[4] = {w[0], w[1], w[2], w[3]}, kb[7][4] = {}, xb[4] = {}, cb[3] = {};
+        step_vjp(ray_k(a, a * a, pt, pph), x, h, ynb, kb, false, xb, cb);
+        auto J = [&](const double* xx, double aa, double ptt, double pphh) {
+            double yn[4];
+            step_fwd(ray_k(aa, aa * aa, ptt, pphh), xx, h, yn);
+            return w[0] * yn[0] + w[1] * yn[1] + w[2] * yn[2] + w[3] * yn[3];
+        };
+        double fd[7];
+        for (int i = 0; i < 4; ++i) {
+            double xp[4] = {x[0], x[1], x[2], x[3]}, xm[4] = {x[0], x[1], x[2], x[3]};
+            const double d = 1e-6 * std::max(1.0, std::fabs(x[i]));
+            xp[i] += d; xm[i] -= d;
+            fd[i] = (J(xp, a, pt, pph) - J(xm, a, pt, pph)) / (2 * d);
+        }
+        const double d = 1e-6;
+        fd[4] = (J(x, a + d, pt, pph) - J(x, a - d, pt, pph)) / (2 * d);
+        fd[5] = (J(x, a, pt + d, pph) - J(x, a, pt - d, pph)) / (2 * d);
+        fd[6] = (J(x, a, pt, pph + d) - J(x, a, pt, pph - d)) / (2 * d);
+        const double ad[7] = {xb[0], xb[1], xb[2], xb[3], cb[0], cb[1], cb[2]};
+        double scale = 0.0;
+        for (int i = 0; i < 7; ++i) scale = std::max(scale, std::fabs(fd[i]));
+        for (int i = 0; i < 7; ++i) worst = std::max(worst, rel(ad[i], fd[i], 1e-3 * scale));
+        ++n;
+    }
+    std::printf("step %d %.3e\n", n, worst);
+}
+
+// r at the equatorial crossing of the dense output of one frozen step.
+static bool cross_r(const RayK& K, const double x[4], double h, double* r_hit, double* s_out) {
+    Ray s{};
+    s.y[1] = x[0]; s.y[2] = x[1]; s.y[4] = x[2]; s.y[5] = x[3];
+    s.pt = K.pt; s.pph = K.pph; s.h = h;
+    Rhs f;
+    kerr_rhs(K, x[0], x[1], x[2], x[3], f);
+    for (int c = 0; c < 6; ++c) s.k1[c] = f.d[c];
+    Att A;
+    attempt(K, s, 1e-9, 1e-11, A);
+    if (!(f.cs * A.g7.cs < 0.0)) return false;
+    double cr[5], ct[5];
+    dense_coeffs(s, A, 1, cr);
+    dense_coeffs(s, A, 2, ct);
+    const double se = root_theta(ct[0], ct[1], ct[2], ct[3], ct[4], f.cs, s.y[2], A.yn[2]);
+    *r_hit = dense_comp(cr[0], cr[1], cr[2], cr[3], cr[4], se);
+    *s_out = se;
+    return true;
+}
+
+static void check_terminal(std::mt19937_64& rng) {
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    double worst = 0.0;
+    int n = 0;
+    for (int t = 0; t < 400 && n < 100; ++t) {
+        const double a = 0.998 * U(rng), pt = -1.0, pph = -4.0 + 8.0 * U(rng);
+        // start just above the equator (theta < pi/2) moving towards larger theta
+        const double x[4] = {6.0 + 6.0 * U(rng), 1.5707963267948966 - 0.001 - 0.004 * U(rng), -0.5 + U(rng),
+                             1.0 + 3.0 * U(rng)};
+        const double h = 0.5 + 1.5 * U(rng);
+        const RayK K = ray_k(a, a * a, pt, pph);
+        double r0, s0;
+        if (!cross_r(K, x, h, &r0, &s0)) continue;
+        double xb[4] = {}, cb[3] = {};
+        terminal_vjp(K, x, h, s0, 1.0, xb, cb);
+        double fd[7];
+        bool ok = true;
+        auto R = [&](const double* xx, double aa, double ptt, double pphh, double* out) {
+            double s_;
+            ok = ok && cross_r(ray_k(aa, aa * aa, ptt, pphh), xx, h, out, &s_);
+        };
+        for (int i = 0; i < 4; ++i) {
+            double xp[4] = {x[0], x[1], x[2], x[3]}, xm[4] = {x[0], x[1], x[2], x[3]};
+            const double d = 1e-6 * std::max(1.0, std::fabs(x[i]));
+            xp[i] += d; xm[i] -= d;
+            double rp, rm;
+            R(xp, a, pt, pph, &rp);
+            R(xm, a, pt, pph, &rm);
+            fd[i] = (rp - rm) / (2 * d);
+        }
+        const double d = 1e-6;
+        double rp, rm;
+        R(x, a + d, pt, pph, &rp); R(x, a - d, pt, pph, &rm); fd[4] = (rp - rm) / (2 * d);
+        R(x, a, pt + d, pph, &rp); R(x, a, pt - d, pph, &rm); fd[5] = (rp - rm) / (2 * d);
+        R(x, a, pt, pph + d, &rp); R(x, a, pt, pph - d, &rm); fd[6] = (rp - rm) / (2 * d);
+        if (!ok) continue;
+        const double ad[7] = {xb[0], xb[1], xb[2], xb[3], cb[0], cb[1], cb[2]};
+        double scale = 0.0;
+        for (int i = 0; i < 7; ++i) scale = std::max(scale, std::fabs(fd[i]));
+        for (int i = 0; i < 7; ++i) worst = std::max(worst, rel(ad[i], fd[i], 1e-3 * scale));
+        ++n;
+    }
+    std::printf("term %d %.3e\n", n, worst);
+}
+
+// Full adaptive forward of one ray: its intensity (0 unless a disc hit).
+static double ray_I(const bhx_scene& sc, int K, int64_t ray, int* cls, int* n_acc) {
+    Frame F;
+    RaygenArgs G;
+    make_frame(sc, K, F, G);
+    std::vector<double> ck(static_cast<size_t>(F.cap) * kCk);
+    Ray s;
+    fwd_init(F, G, ray, s, ck.data());
+    double yo[6], se = 0.0;
+    int term = -1;
+    while (term < 0) term = fwd_step(F, s, ck.data(), yo, &se);
+    *cls = term;
+    *n_acc = s.nsteps;
+    return term == BHX_DISC ? disc_intensity(F, yo[1], s.pt, s.pph) : 0.0;
+}
+
+static void check_ray(double rtol, double atol, int K) {
+    bhx_scene sc{};
+    // a 512^2-like differentiation scene, sampled: a = 0.7, i = 45 deg, r = 500 M
+    sc.M_kg = 10.0 * 1.98847e30; sc.spin = 0.7;
+    sc.cam_r = 500.0; sc.cam_theta = 45.0 * 3.14159265358979323846 / 180.0; sc.cam_fov = 2.0 * std::atan(20.0 / 500.0);
+    sc.width = 64; sc.height = 64; sc.sqrt_spp = 1; sc.max_steps = 20000;
+    sc.disc_enabled = 1; sc.disc_r_out = 20.0; sc.eddington = 0.1; sc.disc_sense = 1;
+    sc.rtol = rtol; sc.atol = atol; sc.r_escape = 3.0 * 500.0 + 100.0; sc.seed = 20240921ull;
+    Frame F;
+    RaygenArgs G;
+    make_frame(sc, K, F, G);
+    std::vector<double> ck(static_cast<size_t>(F.cap) * kCk), seg(static_cast<size_t>(K) * 5);
+    double worst = 0.0;
+    int n = 0, mism = 0;
+    for (int64_t ray = 0; ray < (int64_t)sc.width * sc.height && n < 40; ray += 37) {
+        int cls, n_acc;
+        Ray s;
+        fwd_init(F, G, ray, s, ck.data());
+        double yo[6], se = 0.0;
+        int term = -1;
+        while (term < 0) term = fwd_step(F, s, ck.data(), yo, &se);
+        if (term != BHX_DISC) continue;
+        const double I0 = disc_intensity(F, yo[1], s.pt, s.pph);
+        if (!(I0 > 0.0)) continue;
+        double grad[4] = {0, 0, 0, 0};
+        mism += bwd_ray(F, G, ray, s.nsteps, 1.0, ck.data(), seg.data(), 1, grad);
+        // central differences of the adaptive forward, relative step 1e-6
+        double fd[4];
+        bool ok = true;
+        for (int p = 0; p < 4; ++p) {
+            bhx_scene a = sc, b = sc;
+            double* fa = p == 0 ? &a.spin : p == 1 ? &a.cam_theta : p == 2 ? &a.eddington : &a.cam_r;
+            double* fb = p == 0 ? &b.spin : p == 1 ? &b.cam_theta : p == 2 ? &b.eddington : &b.cam_r;
+            const double d = 1e-6 * std::fabs(*fa);
+            *fa += d; *fb -= d;
+            int ca, cb_, na, nb;
+            const double Ia = ray_I(a, K, ray, &ca, &na), Ib = ray_I(b, K, ray, &cb_, &nb);
+            if (ca != BHX_DISC || cb_ != BHX_DISC) ok = false;
+            fd[p] = (Ia - Ib) / (2 * d);
+        }
+        if (!ok) continue;
+        double e = 0.0;
+        for (int p = 0; p < 4; ++p) e = std::max(e, rel(grad[p], fd[p], 1e-30));
+        if (e > 1e-3) std::printf("  ray %ld steps %d grad %.6e %.6e %.6e %.6e fd %.6e %.6e %.6e %.6e\n", (long)ray,
+                                  s.nsteps, grad[0], grad[1], grad[2], grad[3], fd[0], fd[1], fd[2], fd[3]);
+        worst = std::max(worst, e);
+        ++n;
+    }
+    std::printf("ray_rtol%.0e_K%d %d %.3e mismatch %d\n", rtol, K, n, worst, mism);
+}
+
+int main() {
+    std::mt19937_64 rng(12345);
+    check_rhs(rng);
+    check_step(rng);
+    check_terminal(rng);
+    check_ray(1e-9, 1e-11, 16);
+    check_ray(1e-12, 1e-14, 7);
+    return 0;
+}
