@@ -136,6 +136,107 @@ def batched_leg(tr, torch, rounds=3):
     }
 
 
+def adjoint_leg(tr, torch, width=512, Ks=(16, 64, 256), rounds=3, fd=True, threads=16):
+    """Differentiable disc image at width^2 (a=0.7, i=45 deg, 0.1 Edd, r=500M;
+    target a=0.9, i=50 deg): loss = sum (G*I - G*I_target)^2 with a Gaussian
+    sigma = 2 px.  Times a forward render (the plain path: trace + shade) and
+    a gradient pass (recording forward + adjoint) for each checkpoint interval
+    K; optionally checks the gradient against central differences (h = 1e-6
+    relative) of the oracle's images."""
+    import math
+    import numpy as np
+    from blackhole_amd import scene as S
+    from blackhole_amd.diff import DiscRender, gaussian_blur, with_params
+    P0 = (0.7, math.radians(45.0), 0.1, 500.0)
+    PT = (0.9, math.radians(50.0), 0.1, 500.0)
+    sc = S.disc_frame(width, width)
+    s0 = with_params(sc, P0)
+    dev = "cuda:0"
+    rays = tr.alloc_rays(s0, full=False)
+    out = {"xyz": torch.empty((s0.height, s0.width, 3), dtype=torch.float64, device=dev)}
+    for _ in range(2):
+        tr.trace_async(s0, rays)
+        tr.shade_async(s0, rays, out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(rounds):
+        tr.trace_async(s0, rays)
+        tr.shade_async(s0, rays, out)
+    torch.cuda.synchronize()
+    fwd_ms = (time.perf_counter() - t0) / rounds * 1e3
+    res = {"workload": f"{width}x{width} disc image, a=0.7, i=45 deg, 0.1 Edd, r_obs=500M; target "
+                       f"a=0.9, i=50 deg; L = sum (G*I - G*I_t)^2, Gaussian sigma 2 px",
+           "forward_render_ms": round(fwd_ms, 3), "per_K": {}}
+    best = None
+    w = None
+    for K in Ks:
+        dr = DiscRender(tr, sc, K=K)
+        if w is None:
+            It = dr.forward_raw(PT).view(width, width).clone()
+            I0 = dr.forward_raw(P0).view(width, width).clone().requires_grad_(True)
+            L = ((gaussian_blur(I0) - gaussian_blur(It)) ** 2).sum()
+            L.backward()
+            w = I0.grad.reshape(-1).contiguous()
+            loss = float(L)
+        dr.forward_raw(P0)
+        g = dr.backward_raw(w)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(rounds):
+            dr.forward_raw(P0)
+            g = dr.backward_raw(w)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / rounds * 1e3
+        info = dr.info()
+        res["per_K"][str(K)] = {"grad_pass_ms": round(ms, 3), "record_forward_ms": round(info["fwd_ms"], 3),
+                                "adjoint_ms": round(info["bwd_ms"], 3), "checkpoint_bytes": info["ck_bytes"],
+                                "replay_mismatches": info["mismatches"], "grad_over_fwd": round(ms / fwd_ms, 3)}
+        if best is None or ms < best[1]:
+            best = (K, ms, g)
+        dr.close()
+    res["best_K"] = best[0]
+    res["grad_over_fwd"] = round(best[1] / fwd_ms, 3)
+    res["loss"] = loss
+    res["grad"] = dict(zip(("spin", "cam_theta", "eddington", "cam_r"), (float(v) for v in best[2])))
+    if fd:
+        # central differences of the oracle's images at h = 1e-6 relative (and
+        # h/2 for a Richardson estimate: the photon-ring rays make the image a
+        # strongly curved function of the camera, so the h = 1e-6 difference
+        # quotient itself carries an O(h^2) error of ~1e-5 in the smallest
+        # component), over the pixels whose class is the same in every render
+        import oracle
+        t0 = time.perf_counter()
+        wn = w.cpu().numpy()
+        I_c, c0 = oracle.disc_intensity(s0, threads=threads)
+        dr = DiscRender(tr, sc, K=best[0])
+        dr.forward_raw(P0)
+        stable = c0["cls"] == dr.rays()["cls"]
+        diffs = {}
+        for hr in (1e-6, 5e-7):
+            for k in range(4):
+                h = hr * abs(P0[k])
+                pp, pn = list(P0), list(P0)
+                pp[k] += h
+                pn[k] -= h
+                Ip, cp = oracle.disc_intensity(with_params(sc, pp), threads=threads)
+                In, cn = oracle.disc_intensity(with_params(sc, pn), threads=threads)
+                stable &= (cp["cls"] == c0["cls"]) & (cn["cls"] == c0["cls"])
+                diffs[(hr, k)] = (Ip - In) / (2 * h)
+        fd1 = np.array([np.sum(np.where(stable, wn * diffs[(1e-6, k)], 0.0)) for k in range(4)])
+        fd2 = np.array([np.sum(np.where(stable, wn * diffs[(5e-7, k)], 0.0)) for k in range(4)])
+        fdr = (4.0 * fd2 - fd1) / 3.0
+        wm = torch.from_numpy(np.where(stable, wn, 0.0)).to(dev)
+        ga = dr.backward_raw(wm)
+        dr.close()
+        res["fd_check"] = {"h_rel": 1e-6, "stable_pixels": int(stable.sum()), "pixels": int(stable.size),
+                           "adjoint": ga.tolist(), "oracle_fd": fd1.tolist(),
+                           "rel_err": (np.abs(ga - fd1) / np.abs(fd1)).tolist(),
+                           "oracle_fd_richardson_1e-6_5e-7": fdr.tolist(),
+                           "rel_err_richardson": (np.abs(ga - fdr) / np.abs(fdr)).tolist(),
+                           "seconds": round(time.perf_counter() - t0, 2)}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +245,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--width", type=int, default=1024)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-launch leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation leg")
+    ap.add_argument("--no-fd", action="store_true", help="skip the finite-difference check of the gradient")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 256 MiB device copy first (for HBM-counter calibration)")
     args = ap.parse_args()
@@ -226,6 +329,9 @@ def main():
     if not args.no_batch:
         line["batched"] = batched_leg(tr, torch)
         line["batched_over_plain"] = line["batched"]["batched_over_plain"]
+    if not args.no_adjoint:
+        line["adjoint"] = adjoint_leg(tr, torch, fd=not args.no_fd, threads=min(16, os.cpu_count() or 1))
+        line["grad_over_fwd"] = line["adjoint"]["grad_over_fwd"]
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(sc, threads)

@@ -17,7 +17,12 @@ HIP_SOURCES = ["tracer.hip", "shade.hip", "api.hip", "batch.hip", "adjoint.hip"]
 # tableau, sincos/log/exp coefficients); FP64 VALU ops take no 64-bit
 # literals, so machine LICM would pin every one of them in two registers for
 # the whole loop and spill.  Without it they are rematerialised where used.
-EXTRA = {"tracer.hip": ["-mllvm", "-disable-machine-licm"], "adjoint.hip": ["-mllvm", "-disable-machine-licm"]}
+# adjoint.hip: the recording forward and the backward's replay inline the same
+# DP5 attempt into two kernels and must round identically, so FMAs are formed
+# per source expression only (-ffp-contract=on), never across expressions by
+# the backend, where the choice depends on the surrounding kernel.
+EXTRA = {"tracer.hip": ["-mllvm", "-disable-machine-licm"],
+         "adjoint.hip": ["-mllvm", "-disable-machine-licm", "-ffp-contract=on"]}
 CXX_SOURCES = ["host_setup.cpp", "host_physics.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

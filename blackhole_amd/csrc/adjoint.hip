@@ -17,6 +17,8 @@
 #include "adjoint_frame.h"
 #include "context.h"
 
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -37,10 +39,11 @@ struct AdjArgs {
     double* y;                // 8 x n_rays SoA (layout of bhx_rays.y)
     int32_t* steps;
     double* I;                // n_rays bolometric intensity
+    int32_t* key;             // sort key: accepted steps of a disc hit, else 0
+    unsigned int* n_disc;     // disc hits of the forward
     // backward
     const double* w;          // dL/dI per sample
-    const int32_t* order;     // disc-hit rays, most accepted steps first
-    int32_t n_order;
+    const int32_t* order;     // rays by key, descending: the disc hits first
     double* seg;              // K * 5 doubles per lane, strided by seg_stride
     int64_t seg_stride;
     double* partial;          // 4 per block
@@ -81,9 +84,16 @@ __global__ void __launch_bounds__(256) k_adj_forward(const AdjConst* __restrict_
             P.y[4 * N + ray] = s.pt; P.y[5 * N + ray] = yo[4]; P.y[6 * N + ray] = yo[5]; P.y[7 * N + ray] = s.pph;
             P.steps[ray] = s.nsteps;
             P.I[ray] = (term == BHX_DISC) ? disc_intensity(F, yo[1], s.pt, s.pph) : 0.0;
+            P.key[ray] = (term == BHX_DISC) ? min(s.nsteps, 32767) : 0;   // 15-bit key, >= 1 for a disc hit
+            if (term == BHX_DISC) atomicAdd(P.n_disc, 1u);
             live = false;
         }
     }
+}
+
+__global__ void k_iota(int32_t* p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = static_cast<int32_t>(i);
 }
 
 __global__ void __launch_bounds__(256) k_adj_backward(const AdjConst* __restrict__ C, const AdjArgs P) {
@@ -95,7 +105,8 @@ __global__ void __launch_bounds__(256) k_adj_backward(const AdjConst* __restrict
     const int64_t slab = static_cast<int64_t>(F.cap) * kCk;
     double g[4] = {0.0, 0.0, 0.0, 0.0};
     unsigned int mism = 0;
-    for (int64_t i = lane; i < P.n_order; i += nl) {
+    const int64_t n_order = *P.n_disc;
+    for (int64_t i = lane; i < n_order; i += nl) {
         const int64_t ray = P.order[i];
         const double w = P.w[ray];
         if (w == 0.0) continue;
@@ -125,10 +136,10 @@ struct bhx_adjoint {
     int K = 0;
     int64_t n_rays = 0;
     AdjConst host{};
-    DevBuf cst, ck, cls, y, steps, I, order, seg, partial, counters;
-    std::vector<int32_t> order_h;
+    DevBuf cst, ck, cls, y, steps, I, key, key_sorted, iota, order, sort_tmp, seg, partial, counters;
     int grid_f = 0, grid_b = 0;
     double fwd_ms = 0.0, bwd_ms = 0.0;
+    bool fwd_pending = false;
     unsigned int mismatches = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
@@ -194,13 +205,26 @@ int bhx_adjoint_create(bhx_context* ctx, const bhx_scene* s, int32_t K, bhx_adjo
     if (A->cst.ensure(sizeof(AdjConst)) != hipSuccess || A->ck.ensure(static_cast<size_t>(ck_bytes)) != hipSuccess ||
         A->cls.ensure(n) != hipSuccess || A->y.ensure(sizeof(double) * 8 * n) != hipSuccess ||
         A->steps.ensure(sizeof(int32_t) * n) != hipSuccess || A->I.ensure(sizeof(double) * n) != hipSuccess ||
-        A->order.ensure(sizeof(int32_t) * n) != hipSuccess ||
+        A->order.ensure(sizeof(int32_t) * n) != hipSuccess || A->key.ensure(sizeof(int32_t) * n) != hipSuccess ||
+        A->key_sorted.ensure(sizeof(int32_t) * n) != hipSuccess || A->iota.ensure(sizeof(int32_t) * n) != hipSuccess ||
         A->seg.ensure(sizeof(double) * 5 * K * lanes) != hipSuccess ||
         A->partial.ensure(sizeof(double) * 4 * A->grid_b) != hipSuccess ||
         A->counters.ensure(64) != hipSuccess)
         return bail(bhx_fail(ctx, BHX_E_NOMEM, "adjoint buffers"));
     for (auto& e : A->ev)
         if (hipEventCreate(&e) != hipSuccess) return bail(bhx_fail(ctx, BHX_E_HIP, "hipEventCreate"));
+    // ray ids 0..n-1 (sort values) and the radix sort's scratch
+    hipLaunchKernelGGL(k_iota, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, 0,
+                       static_cast<int32_t*>(A->iota.p), n);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+        return bail(bhx_fail(ctx, BHX_E_HIP, "k_iota"));
+    size_t tmp = 0;
+    if (hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp, static_cast<const int32_t*>(A->key.p),
+                                                     static_cast<int32_t*>(A->key_sorted.p),
+                                                     static_cast<const int32_t*>(A->iota.p),
+                                                     static_cast<int32_t*>(A->order.p), n, 0, 15) != hipSuccess ||
+        A->sort_tmp.ensure(std::max<size_t>(tmp, 16)) != hipSuccess)
+        return bail(bhx_fail(ctx, BHX_E_NOMEM, "radix sort scratch"));
     const int rc = adj_set_scene(A, s);
     if (rc) return bail(rc);
     *out = A;
@@ -211,8 +235,8 @@ int bhx_adjoint_destroy(bhx_adjoint* A) {
     if (!A) return BHX_OK;
     (void)hipSetDevice(A->ctx->device);
     (void)hipDeviceSynchronize();
-    for (DevBuf* d : {&A->cst, &A->ck, &A->cls, &A->y, &A->steps, &A->I, &A->order, &A->seg, &A->partial,
-                      &A->counters})
+    for (DevBuf* d : {&A->cst, &A->ck, &A->cls, &A->y, &A->steps, &A->I, &A->key, &A->key_sorted, &A->iota,
+                      &A->order, &A->sort_tmp, &A->seg, &A->partial, &A->counters})
         d->release();
     for (auto& e : A->ev)
         if (e) (void)hipEventDestroy(e);
@@ -240,29 +264,23 @@ int bhx_adjoint_forward(bhx_adjoint* A, double* I, void* stream) {
     P.y = static_cast<double*>(A->y.p);
     P.steps = static_cast<int32_t*>(A->steps.p);
     P.I = I ? I : static_cast<double*>(A->I.p);
+    P.key = static_cast<int32_t*>(A->key.p);
+    P.n_disc = reinterpret_cast<unsigned int*>(static_cast<char*>(A->counters.p) + 8);
+    // counters: [0] queue, [8] disc hits, [32] replay mismatches
     HIPCHK(launch_zero_u64(static_cast<unsigned long long*>(A->counters.p), 8, st));
     HIPCHK(hipEventRecord(A->ev[0], st));
     hipLaunchKernelGGL(k_adj_forward, dim3(A->grid_f), dim3(256), 0, st,
                        static_cast<const AdjConst*>(A->cst.p), P);
     HIPCHK(hipGetLastError());
+    // the backward's ray order: disc hits first, most accepted steps first
+    // (a stable radix sort on the 15-bit step count: deterministic)
+    size_t tmp = A->sort_tmp.bytes;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairsDescending(A->sort_tmp.p, tmp, static_cast<const int32_t*>(A->key.p),
+                                                        static_cast<int32_t*>(A->key_sorted.p),
+                                                        static_cast<const int32_t*>(A->iota.p),
+                                                        static_cast<int32_t*>(A->order.p), A->n_rays, 0, 15, st));
     HIPCHK(hipEventRecord(A->ev[1], st));
-    // the backward's ray order: disc hits, most accepted steps first
-    std::vector<uint8_t> cls(A->n_rays);
-    std::vector<int32_t> steps(A->n_rays);
-    HIPCHK(hipMemcpyAsync(cls.data(), A->cls.p, A->n_rays, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(steps.data(), A->steps.p, sizeof(int32_t) * A->n_rays, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    A->order_h.clear();
-    for (int64_t r = 0; r < A->n_rays; ++r)
-        if (cls[r] == BHX_DISC) A->order_h.push_back(static_cast<int32_t>(r));
-    std::stable_sort(A->order_h.begin(), A->order_h.end(),
-                     [&](int32_t a, int32_t b) { return steps[a] > steps[b]; });
-    if (!A->order_h.empty())
-        HIPCHK(hipMemcpy(A->order.p, A->order_h.data(), sizeof(int32_t) * A->order_h.size(),
-                         hipMemcpyHostToDevice));
-    A->fwd_ms = 0.0;
-    float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, A->ev[0], A->ev[1]) == hipSuccess) A->fwd_ms = ms;
+    A->fwd_pending = true;
     return BHX_OK;
 }
 
@@ -277,12 +295,12 @@ int bhx_adjoint_backward(bhx_adjoint* A, const double* w, double* grad, void* st
     P.steps = static_cast<int32_t*>(A->steps.p);
     P.w = w;
     P.order = static_cast<const int32_t*>(A->order.p);
-    P.n_order = static_cast<int32_t>(A->order_h.size());
     P.seg = static_cast<double*>(A->seg.p);
     P.seg_stride = static_cast<int64_t>(A->grid_b) * 256;
     P.partial = static_cast<double*>(A->partial.p);
     P.mismatch = reinterpret_cast<unsigned int*>(static_cast<char*>(A->counters.p) + 32);
-    HIPCHK(launch_zero_u64(static_cast<unsigned long long*>(A->counters.p), 8, st));
+    P.n_disc = reinterpret_cast<unsigned int*>(static_cast<char*>(A->counters.p) + 8);
+    HIPCHK(launch_zero_u32(P.mismatch, 1, st));
     HIPCHK(hipEventRecord(A->ev[2], st));
     hipLaunchKernelGGL(k_adj_backward, dim3(A->grid_b), dim3(256), 0, st,
                        static_cast<const AdjConst*>(A->cst.p), P);
@@ -299,6 +317,8 @@ int bhx_adjoint_backward(bhx_adjoint* A, const double* w, double* grad, void* st
     A->mismatches = mism;
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, A->ev[2], A->ev[3]) == hipSuccess) A->bwd_ms = ms;
+    if (A->fwd_pending && hipEventElapsedTime(&ms, A->ev[0], A->ev[1]) == hipSuccess) A->fwd_ms = ms;
+    A->fwd_pending = false;
     return BHX_OK;
 }
 
@@ -322,11 +342,20 @@ int bhx_adjoint_copy_rays(bhx_adjoint* A, uint8_t* cls, double* y, int32_t* step
     return BHX_OK;
 }
 
-int bhx_adjoint_info(const bhx_adjoint* A, int64_t* ck_bytes, int64_t* disc_rays, int64_t* mismatches,
+int bhx_adjoint_info(bhx_adjoint* A, int64_t* ck_bytes, int64_t* disc_rays, int64_t* mismatches,
                      double* fwd_ms, double* bwd_ms) {
     if (!A) return BHX_E_INVALID;
+    bhx_context* ctx = A->ctx;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipDeviceSynchronize());
+    if (A->fwd_pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, A->ev[0], A->ev[1]) == hipSuccess) A->fwd_ms = ms;
+    }
+    unsigned int nd = 0;
+    HIPCHK(hipMemcpy(&nd, static_cast<char*>(A->counters.p) + 8, sizeof nd, hipMemcpyDeviceToHost));
     if (ck_bytes) *ck_bytes = static_cast<int64_t>(A->ck.bytes);
-    if (disc_rays) *disc_rays = static_cast<int64_t>(A->order_h.size());
+    if (disc_rays) *disc_rays = nd;
     if (mismatches) *mismatches = A->mismatches;
     if (fwd_ms) *fwd_ms = A->fwd_ms;
     if (bwd_ms) *bwd_ms = A->bwd_ms;

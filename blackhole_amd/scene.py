@@ -168,12 +168,17 @@ def survey_batch(seed=20240921, n=64, n_ns=8, side_min=128, side_max=1024):
     return out
 
 
-def disc_frame(width=512, height=512, spin=0.7, incl_deg=45.0, eddington=0.1, r_cam=500.0):
+def disc_frame(width=512, height=512, spin=0.7, incl_deg=45.0, eddington=0.1, r_cam=500.0,
+               rtol=1e-10, atol=1e-12):
     """The differentiation scene: NT disc ISCO-20M, no stars, camera at r_cam
-    (default 500 M) and inclination incl_deg over a +-20 M image plane, CLI
-    tolerances; r_escape fixed at 3 * 500 + 100 so that varying r_cam does not
-    move the escape sphere."""
+    (default 500 M) and inclination incl_deg over a +-20 M image plane;
+    r_escape fixed at 3 * 500 + 100 so that varying r_cam does not move the
+    escape sphere.  Tolerances 1e-10 / 1e-12: the derivative of the
+    integrator with its step sizes frozen approaches the converged derivative
+    like the tolerance to a power above one (at 1e-9 single rays differ from
+    it by ~1e-5, at 1e-10 by ~4e-7)."""
     s = plain_scene(spin=spin, incl_deg=incl_deg, r_cam=r_cam, width=width, height=height,
-                    fov=2.0 * math.atan(20.0 / 500.0), disc_r_out=20.0, eddington=eddington)
+                    fov=2.0 * math.atan(20.0 / 500.0), disc_r_out=20.0, eddington=eddington,
+                    rtol=rtol, atol=atol)
     s.r_escape = 3.0 * 500.0 + 100.0
     return s

@@ -275,8 +275,9 @@ int bhx_adjoint_create(bhx_context *ctx, const bhx_scene *scene, int32_t K, bhx_
 int bhx_adjoint_destroy(bhx_adjoint *adj);
 /* New parameters for the same frame size and max_steps (synchronises). */
 int bhx_adjoint_set_scene(bhx_adjoint *adj, const bhx_scene *scene);
-/* Recording forward: I (device, one double per sample) and the checkpoints;
- * synchronises `stream` (the backward's ray order is built on the host). */
+/* Recording forward: I (device, one double per sample) and the checkpoints,
+ * then the backward's ray order (disc hits by step count, a device radix
+ * sort).  Stream-ordered, no host synchronisation. */
 int bhx_adjoint_forward(bhx_adjoint *adj, double *I, void *stream);
 /* w (device, one double per sample) = dL/dI; grad (host, 4 doubles) =
  * dL/d(spin, cam_theta, eddington, cam_r).  Synchronous. */
@@ -287,7 +288,7 @@ int bhx_adjoint_rays(const bhx_adjoint *adj, bhx_rays *rays);
 int bhx_adjoint_copy_rays(bhx_adjoint *adj, uint8_t *cls, double *y, int32_t *steps, void *stream);
 /* Checkpoint slab size, disc-hit rays of the last forward, replayed segments
  * that did not reproduce their checkpoint (0 expected), kernel times. */
-int bhx_adjoint_info(const bhx_adjoint *adj, int64_t *ck_bytes, int64_t *disc_rays, int64_t *mismatches,
+int bhx_adjoint_info(bhx_adjoint *adj, int64_t *ck_bytes, int64_t *disc_rays, int64_t *mismatches,
                      double *fwd_ms, double *bwd_ms);
 
 /* ---- scene preparation (host only, no context, no GPU) ------------------ */

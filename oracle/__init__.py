@@ -200,6 +200,19 @@ def ref_trace_list(scene, ray_ids):
     return out
 
 
+def disc_intensity(scene, threads=0):
+    """Every ray of the scene traced by the oracle, and the bolometric
+    intensity sigma (g T)^4 / pi of its disc hits (0 elsewhere)."""
+    ids = np.arange(scene.n_rays)
+    c = trace_list(scene, ids, threads=threads)
+    cls = np.ascontiguousarray(c["cls"], dtype=np.uint8)
+    r = np.ascontiguousarray(c["y"][1])
+    g = np.ascontiguousarray(c["g"])
+    out = np.zeros(scene.n_rays)
+    lib().oracle_disc_intensity(C.byref(scene), C.c_int64(scene.n_rays), _ptr(cls), _ptr(r), _ptr(g), _ptr(out))
+    return out, c
+
+
 def auto_exposure(xyz, key=1.0):
     x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
     return lib().oracle_auto_exposure(_ptr(x), len(x), key)

@@ -1327,6 +1327,22 @@ double oracle_disc_temperature(const bhx_scene* s, double r) {
     const Disc d(s->M_kg, s->spin, s->disc_r_out, s->eddington, s->disc_sense);
     return d.temperature(r);
 }
+// Bolometric intensity sigma (g T)^4 / pi of the disc hits among n rays
+// (class 1 with g > 0 and T > 0), else 0: the loss input of the
+// differentiable disc image.
+void oracle_disc_intensity(const bhx_scene* s, int64_t n, const uint8_t* cls, const double* r, const double* g,
+                           double* out) {
+    const Disc d(s->M_kg, s->spin, s->disc_r_out, s->eddington, s->disc_sense);
+    const double sigma = 5.670374419e-8, pi = 3.14159265358979323846;
+    for (int64_t i = 0; i < n; ++i) {
+        out[i] = 0.0;
+        if (cls[i] != 1 || !(g[i] > 0.0)) continue;
+        const double T = d.temperature(r[i]);
+        if (!(T > 0.0)) continue;
+        const double gT = g[i] * T;
+        out[i] = sigma * (gT * gT) * (gT * gT) / pi;
+    }
+}
 void oracle_blackbody_xyz(double T, double* o) { bb_xyz(T, o); }
 void oracle_blackbody_xyz_exact(double T, double* o) { bb_exact(T, o); }
 void oracle_powerlaw_xyz(double alpha, double* o) { powerlaw(alpha, o); }

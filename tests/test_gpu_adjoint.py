@@ -9,7 +9,6 @@
   the same in all nine renders (the smooth part of the image);
 * the torch.autograd.Function gives the same gradient as the raw backward.
 """
-import ctypes as C
 import math
 
 import numpy as np
@@ -23,7 +22,6 @@ from parity import compare
 
 pytestmark = pytest.mark.gpu
 
-SIGMA_SB = 5.670374419e-8
 P0 = (0.7, math.radians(45.0), 0.1, 500.0)
 PT = (0.9, math.radians(50.0), 0.1, 500.0)
 
@@ -32,17 +30,9 @@ def _scene(n):
     return S.disc_frame(n, n)
 
 
-def oracle_intensity(sc, cat=None):
+def oracle_intensity(sc):
     """I per ray from the oracle: sigma (g T(r))^4 / pi on disc hits."""
-    ids = np.arange(sc.n_rays)
-    c = oracle.trace_list(sc, ids)
-    I = np.zeros(sc.n_rays)
-    L = oracle.lib()
-    for i in np.nonzero(c["cls"] == 1)[0]:
-        T = L.oracle_disc_temperature(C.byref(sc), C.c_double(float(c["y"][1][i])))
-        if T > 0.0 and c["g"][i] > 0.0:
-            I[i] = SIGMA_SB * (c["g"][i] * T) ** 4 / math.pi
-    return I, c
+    return oracle.disc_intensity(sc)
 
 
 def test_forward_matches_oracle(tracer):
