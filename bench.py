@@ -15,12 +15,16 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# Algorithmic FP64 operation counts of the reference's path (hand count of
-# src/geodesic.h / src/kerr.h / src/render.cpp): per accepted DP5 step
-# (6 RHS + stage/error/controller + dense output + per-step invariants and
-# events) and per rejected attempt.
-ALG_FLOPS_PER_ACCEPTED_STEP = 1503
-ALG_FLOPS_PER_REJECTED_ATTEMPT = 1280
+# Algorithmic FP64 operation counts of the reference's path (src/geodesic.h,
+# src/kerr.h, src/render.cpp), counted by instantiating the oracle's
+# restatement with an operation-counting scalar (+ - * / sqrt and each
+# transcendental = 1; tests/test_flop_counts.py checks these against it):
+# per RHS evaluation; per accepted step (6 RHS + stages, error norm,
+# controller, dense output + the loop's invariants, end point, hmax and disc
+# sign test); per rejected attempt (6 RHS + stages, error norm, controller).
+ALG_FLOPS_PER_RHS = 151
+ALG_FLOPS_PER_ACCEPTED_STEP = 1659
+ALG_FLOPS_PER_REJECTED_ATTEMPT = 1417
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector peak (AMD published; FP64 matrix peak is the same)
 # HBM bytes per k_trace launch at this workload, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs scaled by a copy kernel of known size (the file says how).

@@ -40,18 +40,32 @@ double eddington_lum(double M) { return 4.0 * kPi * kG * M * kMp * kC / kSigmaT;
 double eddington_mdot(double M, double eta) { return eddington_lum(M) / (eta * kC * kC); }  // :62-64
 
 // ---- Kerr geometry (src/kerr.h:37-176) ------------------------------------
-struct Pt {                    // shared sub-expressions at (r, theta)
-    double a, a2, r, sn, cs, sn2, cs2, S, D, A;
+// The geometry, the Hamiltonian flow, the DP5 attempt and the per-step
+// invariants are templates over the scalar type: double everywhere in the
+// oracle, and a FLOP-counting type for oracle_flop_counts (the algorithmic
+// operation counts of the roofline).  T = double compiles to the same
+// arithmetic as plain functions.
+template <class T>
+struct PtT {                   // shared sub-expressions at (r, theta)
+    T a, a2, r, sn, cs, sn2, cs2, S, D, A;
 };
+using Pt = PtT<double>;
 
-Pt point(double a, double r, double th) {  // kerr.h:44-59 (sin clamp kerr.h:34,51)
-    Pt p;
+// std::max / std::min / std::clamp with the same comparisons, for any T
+template <class T, class U> T tmax(const T& x, const U& y) { return (x < y) ? T(y) : x; }
+template <class T, class U> T tmin(const T& x, const U& y) { return (y < x) ? T(y) : x; }
+template <class T> T tclamp(const T& v, double lo, double hi) { return (v < lo) ? T(lo) : ((hi < v) ? T(hi) : v); }
+
+template <class T>
+PtT<T> point_t(T a, T r, T th) {  // kerr.h:44-59 (sin clamp kerr.h:34,51)
+    using std::cos; using std::fabs; using std::sin;
+    PtT<T> p;
     p.a = a;
     p.a2 = a * a;
     p.r = r;
-    p.sn = std::sin(th);
-    if (std::fabs(p.sn) < 1e-9) p.sn = (p.sn < 0.0) ? -1e-9 : 1e-9;
-    p.cs = std::cos(th);
+    p.sn = sin(th);
+    if (fabs(p.sn) < 1e-9) p.sn = (p.sn < 0.0) ? T(-1e-9) : T(1e-9);
+    p.cs = cos(th);
     p.sn2 = p.sn * p.sn;
     p.cs2 = p.cs * p.cs;
     p.S = r * r + p.a2 * p.cs2;
@@ -59,11 +73,13 @@ Pt point(double a, double r, double th) {  // kerr.h:44-59 (sin clamp kerr.h:34,
     p.A = (r * r + p.a2) * (r * r + p.a2) - p.a2 * p.D * p.sn2;
     return p;
 }
+inline Pt point(double a, double r, double th) { return point_t<double>(a, r, th); }
 
 // Index order of the five non-zero components: tt, tp, rr, hh (theta-theta), pp.
 enum { TT = 0, TP = 1, RR = 2, HH = 3, PP = 4 };
 
-void cov_metric(const Pt& p, double g[5]) {  // kerr.h:69-77
+template <class T>
+void cov_metric(const PtT<T>& p, T g[5]) {  // kerr.h:69-77
     g[TT] = -(1.0 - 2.0 * p.r / p.S);
     g[TP] = -2.0 * p.a * p.r * p.sn2 / p.S;
     g[RR] = p.S / p.D;
@@ -71,8 +87,9 @@ void cov_metric(const Pt& p, double g[5]) {  // kerr.h:69-77
     g[PP] = p.A * p.sn2 / p.S;
 }
 
-void inv_metric(const Pt& p, double g[5]) {  // kerr.h:107-116
-    const double SD = p.S * p.D;
+template <class T>
+void inv_metric(const PtT<T>& p, T g[5]) {  // kerr.h:107-116
+    const T SD = p.S * p.D;
     g[TT] = -p.A / SD;
     g[TP] = -2.0 * p.a * p.r / SD;
     g[RR] = p.D / p.S;
@@ -81,29 +98,30 @@ void inv_metric(const Pt& p, double g[5]) {  // kerr.h:107-116
 }
 
 // Inverse metric and its r, theta partials, closed form (kerr.h:129-176).
-void inv_metric_partials(const Pt& p, double g[5], double gr[5], double gh[5]) {
+template <class T>
+void inv_metric_partials(const PtT<T>& p, T g[5], T gr[5], T gh[5]) {
     inv_metric(p, g);
-    const double r = p.r, a2 = p.a2, a = p.a, sn = p.sn, cs = p.cs, sn2 = p.sn2;
-    const double S = p.S, D = p.D, A = p.A;
-    const double Sr = 2.0 * r, Sh = -2.0 * a2 * sn * cs;
-    const double Dr = 2.0 * r - 2.0, Dh = 0.0;
-    const double Ar = 4.0 * r * (r * r + a2) - a2 * Dr * sn2;
-    const double Ah = -2.0 * a2 * D * sn * cs;
-    const double Q = S * D;
-    const double Qr = Sr * D + S * Dr;
-    const double Qh = Sh * D + S * Dh;
-    const double Q2 = Q * Q;
+    const T r = p.r, a2 = p.a2, a = p.a, sn = p.sn, cs = p.cs, sn2 = p.sn2;
+    const T S = p.S, D = p.D, A = p.A;
+    const T Sr = 2.0 * r, Sh = -2.0 * a2 * sn * cs;
+    const T Dr = 2.0 * r - 2.0, Dh = 0.0;
+    const T Ar = 4.0 * r * (r * r + a2) - a2 * Dr * sn2;
+    const T Ah = -2.0 * a2 * D * sn * cs;
+    const T Q = S * D;
+    const T Qr = Sr * D + S * Dr;
+    const T Qh = Sh * D + S * Dh;
+    const T Q2 = Q * Q;
     gr[TT] = -(Ar * Q - A * Qr) / Q2;
     gh[TT] = -(Ah * Q - A * Qh) / Q2;
-    const double N = -2.0 * a * r, Nr = -2.0 * a, Nh = 0.0;
+    const T N = -2.0 * a * r, Nr = -2.0 * a, Nh = 0.0;
     gr[TP] = (Nr * Q - N * Qr) / Q2;
     gh[TP] = (Nh * Q - N * Qh) / Q2;
     gr[RR] = (Dr * S - D * Sr) / (S * S);
     gh[RR] = (Dh * S - D * Sh) / (S * S);
     gr[HH] = -Sr / (S * S);
     gh[HH] = -Sh / (S * S);
-    const double P = D - a2 * sn2, Pr = Dr, Ph = -2.0 * a2 * sn * cs;
-    const double Rq = Q * sn2, Rr = Qr * sn2, Rh = Qh * sn2 + Q * 2.0 * sn * cs;
+    const T P = D - a2 * sn2, Pr = Dr, Ph = -2.0 * a2 * sn * cs;
+    const T Rq = Q * sn2, Rr = Qr * sn2, Rh = Qh * sn2 + Q * 2.0 * sn * cs;
     gr[PP] = (Pr * Rq - P * Rr) / (Rq * Rq);
     gh[PP] = (Ph * Rq - P * Rh) / (Rq * Rq);
 }
@@ -154,10 +172,12 @@ void zamo(const Pt& p, double e[4][4]) {
     e[3][3] = std::sqrt(p.S / p.A) / p.sn;
 }
 
-double carter_Q(double a, double th, double pth, double E, double Lz) {  // kerr.h:342-348, m2 = 0
-    double sn = std::sin(th);
-    if (std::fabs(sn) < 1e-9) sn = 1e-9;
-    const double cs2 = std::cos(th) * std::cos(th);
+template <class T>
+T carter_Q(T a, T th, T pth, T E, T Lz) {  // kerr.h:342-348, m2 = 0
+    using std::cos; using std::fabs; using std::sin;
+    T sn = sin(th);
+    if (fabs(sn) < 1e-9) sn = T(1e-9);
+    const T cs2 = cos(th) * cos(th);
     return pth * pth + cs2 * (a * a * (0.0 - E * E) + Lz * Lz / (sn * sn));
 }
 
@@ -165,22 +185,25 @@ double carter_Q(double a, double th, double pth, double E, double Lz) {  // kerr
 // y = (t, r, th, ph, p_t, p_r, p_th, p_ph)
 enum { YT = 0, YR = 1, YH = 2, YP = 3, YPT = 4, YPR = 5, YPH = 6, YPP = 7 };
 
-void rhs(double a, const double y[8], double d[8]) {
-    const Pt p = point(a, y[YR], y[YH]);
-    double g[5], gr[5], gh[5];
+template <class T>
+void rhs_t(T a, const T y[8], T d[8]) {
+    const PtT<T> p = point_t<T>(a, y[YR], y[YH]);
+    T g[5], gr[5], gh[5];
     inv_metric_partials(p, g, gr, gh);
-    const double pt = y[YPT], pr = y[YPR], ph = y[YPH], pp = y[YPP];
+    const T pt = y[YPT], pr = y[YPR], ph = y[YPH], pp = y[YPP];
     d[YT] = g[TT] * pt + g[TP] * pp;
     d[YR] = g[RR] * pr;
     d[YH] = g[HH] * ph;
     d[YP] = g[TP] * pt + g[PP] * pp;
-    d[YPT] = 0.0;
-    d[YPP] = 0.0;
+    d[YPT] = T(0.0);
+    d[YPP] = T(0.0);
     d[YPR] = -0.5 * (gr[TT] * pt * pt + 2.0 * gr[TP] * pt * pp + gr[RR] * pr * pr +
                      gr[HH] * ph * ph + gr[PP] * pp * pp);
     d[YPH] = -0.5 * (gh[TT] * pt * pt + 2.0 * gh[TP] * pt * pp + gh[RR] * pr * pr +
                      gh[HH] * ph * ph + gh[PP] * pp * pp);
 }
+
+inline void rhs(double a, const double y[8], double d[8]) { rhs_t<double>(a, y, d); }
 
 // ---- Dormand-Prince 5(4) coefficients (Hairer, Norsett & Wanner II.4) ----
 // Same literal fractions as geodesic.h:77-106 (they are the published tableau).
@@ -203,9 +226,10 @@ constexpr double d1 = -12715105075.0 / 11282082432.0, d3 = 87487479700.0 / 32700
 }  // namespace dp5
 
 // Hairer's dense-output polynomial of one accepted step (geodesic.h:109-121).
-struct Seg {
-    double c1[8], c2[8], c3[8], c4[8], c5[8];
-    double h;
+template <class T>
+struct SegT {
+    T c1[8], c2[8], c3[8], c4[8], c5[8];
+    T h;
     void at(double s, double out[8]) const {
         const double s1 = 1.0 - s;
         for (int i = 0; i < 8; ++i)
@@ -216,54 +240,59 @@ struct Seg {
         return c1[i] + s * (c2[i] + s1 * (c3[i] + s * (c4[i] + s1 * c5[i])));
     }
 };
+using Seg = SegT<double>;
 
 // One integrator instance: FSAL first stage + PI-controller memory
 // (geodesic.h:123-204).
-struct DP5 {
-    double a, rtol, atol;
-    double k1[8];
+inline bool finite_t(double x) { return std::isfinite(x); }
+
+template <class T>
+struct DP5T {
+    T a, rtol, atol;
+    T k1[8];
     bool k1_ok = false;
-    double err_prev = 1e-4;
+    T err_prev = T(1e-4);
     long long rhs_calls = 0;
 
-    void reset() { k1_ok = false; err_prev = 1e-4; }
+    void reset() { k1_ok = false; err_prev = T(1e-4); }
 
-    bool attempt(const double y[8], double& h, double ynew[8], Seg& seg) {
+    bool attempt(const T y[8], T& h, T ynew[8], SegT<T>& seg) {
         using namespace dp5;
-        if (!k1_ok) { rhs(a, y, k1); k1_ok = true; ++rhs_calls; }
-        double t[8], k2[8], k3[8], k4[8], k5[8], k6[8], k7[8];
+        using std::fabs; using std::pow; using std::sqrt;
+        if (!k1_ok) { rhs_t<T>(a, y, k1); k1_ok = true; ++rhs_calls; }
+        T t[8], k2[8], k3[8], k4[8], k5[8], k6[8], k7[8];
         for (int i = 0; i < 8; ++i) t[i] = y[i] + h * a21 * k1[i];
-        rhs(a, t, k2);
+        rhs_t<T>(a, t, k2);
         for (int i = 0; i < 8; ++i) t[i] = y[i] + h * (a31 * k1[i] + a32 * k2[i]);
-        rhs(a, t, k3);
+        rhs_t<T>(a, t, k3);
         for (int i = 0; i < 8; ++i) t[i] = y[i] + h * (a41 * k1[i] + a42 * k2[i] + a43 * k3[i]);
-        rhs(a, t, k4);
+        rhs_t<T>(a, t, k4);
         for (int i = 0; i < 8; ++i)
             t[i] = y[i] + h * (a51 * k1[i] + a52 * k2[i] + a53 * k3[i] + a54 * k4[i]);
-        rhs(a, t, k5);
+        rhs_t<T>(a, t, k5);
         for (int i = 0; i < 8; ++i)
             t[i] = y[i] + h * (a61 * k1[i] + a62 * k2[i] + a63 * k3[i] + a64 * k4[i] + a65 * k5[i]);
-        rhs(a, t, k6);
+        rhs_t<T>(a, t, k6);
         for (int i = 0; i < 8; ++i)
             ynew[i] = y[i] + h * (b1 * k1[i] + b3 * k3[i] + b4 * k4[i] + b5 * k5[i] + b6 * k6[i]);
-        rhs(a, ynew, k7);
+        rhs_t<T>(a, ynew, k7);
         rhs_calls += 6;
 
-        double acc = 0.0;
+        T acc = T(0.0);
         for (int i = 0; i < 8; ++i) {
-            const double e = h * (e1 * k1[i] + e3 * k3[i] + e4 * k4[i] + e5 * k5[i] + e6 * k6[i] +
-                                  e7 * k7[i]);
-            const double sc = atol + rtol * std::max(std::fabs(y[i]), std::fabs(ynew[i]));
-            const double q = e / sc;
+            const T e = h * (e1 * k1[i] + e3 * k3[i] + e4 * k4[i] + e5 * k5[i] + e6 * k6[i] +
+                             e7 * k7[i]);
+            const T sc = atol + rtol * tmax(fabs(y[i]), fabs(ynew[i]));
+            const T q = e / sc;
             acc += q * q;
         }
-        double err = std::sqrt(acc / 8.0);
-        if (!std::isfinite(err)) err = 1e10;
+        T err = sqrt(acc / 8.0);
+        if (!finite_t(err)) err = T(1e10);
 
         const double alpha = 0.7 / 5.0, beta = 0.4 / 5.0, safety = 0.9;
-        double fac = safety * std::pow(std::max(err, 1e-16), -alpha) *
-                     std::pow(std::max(err_prev, 1e-4), beta);
-        fac = std::clamp(fac, 0.2, 5.0);
+        T fac = safety * pow(tmax(err, 1e-16), -alpha) *
+                pow(tmax(err_prev, 1e-4), beta);
+        fac = tclamp(fac, 0.2, 5.0);
 
         if (err <= 1.0) {
             seg.h = h;
@@ -276,14 +305,15 @@ struct DP5 {
                                  d7 * k7[i]);
             }
             for (int i = 0; i < 8; ++i) k1[i] = k7[i];
-            err_prev = std::max(err, 1e-4);
+            err_prev = tmax(err, 1e-4);
             h *= fac;
             return true;
         }
-        h *= std::min(fac, 1.0);
+        h *= tmin(fac, 1.0);
         return false;
     }
 };
+using DP5 = DP5T<double>;
 
 // Null momentum from a local direction in the ZAMO frame, past-directed
 // (geodesic.h:215-231 with past_directed = true).
@@ -304,31 +334,41 @@ void photon(double a, double r, double th, double phi, const double e[4][4], con
     y[YPP] = g[TP] * pu[0] + g[PP] * pu[3];
 }
 
-struct Inv { double E, Lz, Q, norm, norm_rel; };
+template <class T>
+struct InvT { T E, Lz, Q, norm, norm_rel; };
+using Inv = InvT<double>;
 
-Inv invariants(double a, const double y[8]) {  // geodesic.h:240-251, kerr.h:301-321
-    Inv v;
+template <class T>
+InvT<T> invariants_t(T a, const T y[8]) {  // geodesic.h:240-251, kerr.h:301-321
+    using std::fabs;
+    InvT<T> v;
     v.E = -y[YPT];
     v.Lz = y[YPP];
-    v.Q = carter_Q(a, y[YH], y[YPH], v.E, v.Lz);
-    double g[5];
-    inv_metric(point(a, y[YR], y[YH]), g);
-    const double p0 = y[YPT], p1 = y[YPR], p2 = y[YPH], p3 = y[YPP];
+    v.Q = carter_Q<T>(a, y[YH], y[YPH], v.E, v.Lz);
+    T g[5];
+    inv_metric(point_t<T>(a, y[YR], y[YH]), g);
+    const T p0 = y[YPT], p1 = y[YPR], p2 = y[YPH], p3 = y[YPP];
     v.norm = g[TT] * p0 * p0 + 2.0 * g[TP] * p0 * p3 + g[RR] * p1 * p1 + g[HH] * p2 * p2 +
              g[PP] * p3 * p3;
-    const double sc = std::fabs(g[TT] * p0 * p0) + std::fabs(2.0 * g[TP] * p0 * p3) +
-                      std::fabs(g[RR] * p1 * p1) + std::fabs(g[HH] * p2 * p2) +
-                      std::fabs(g[PP] * p3 * p3);
+    const T sc = fabs(g[TT] * p0 * p0) + fabs(2.0 * g[TP] * p0 * p3) +
+                 fabs(g[RR] * p1 * p1) + fabs(g[HH] * p2 * p2) +
+                 fabs(g[PP] * p3 * p3);
     v.norm_rel = (sc > 0.0) ? v.norm / sc : v.norm;
     return v;
 }
+inline Inv invariants(double a, const double y[8]) { return invariants_t<double>(a, y); }
 
-void to_cartesian(double a, double r, double th, double ph, double o[3]) {  // geodesic.h:256-262
-    const double rr = std::sqrt(r * r + a * a);
-    const double sn = std::sin(th);
-    o[0] = rr * sn * std::cos(ph);
-    o[1] = rr * sn * std::sin(ph);
-    o[2] = r * std::cos(th);
+template <class T>
+void to_cartesian_t(T a, T r, T th, T ph, T o[3]) {  // geodesic.h:256-262
+    using std::cos; using std::sin; using std::sqrt;
+    const T rr = sqrt(r * r + a * a);
+    const T sn = sin(th);
+    o[0] = rr * sn * cos(ph);
+    o[1] = rr * sn * sin(ph);
+    o[2] = r * cos(th);
+}
+inline void to_cartesian(double a, double r, double th, double ph, double o[3]) {
+    to_cartesian_t<double>(a, r, th, ph, o);
 }
 
 // ---- spectrum (src/spectrum.cpp) -----------------------------------------
@@ -1135,7 +1175,104 @@ int n_threads(int req) {  // default: all cores, at most 16 (one GPU host's CPU 
 
 }  // namespace
 
+namespace {
+
+// ---- algorithmic operation counts -------------------------------------------
+// A scalar that counts its arithmetic: +, -, *, / and sqrt are one operation
+// each, and so is each transcendental (sin, cos, pow); comparisons, fabs,
+// max / min / clamp and negation are free.  Instantiating the templates
+// above with it counts the reference's expressions as written.
+struct Flop {
+    double v = 0.0;
+    Flop() = default;
+    Flop(double x) : v(x) {}  // NOLINT: constants promote
+};
+thread_local long long g_flops = 0;
+inline Flop op(double v) { ++g_flops; return Flop(v); }
+inline Flop operator+(Flop a, Flop b) { return op(a.v + b.v); }
+inline Flop operator-(Flop a, Flop b) { return op(a.v - b.v); }
+inline Flop operator*(Flop a, Flop b) { return op(a.v * b.v); }
+inline Flop operator/(Flop a, Flop b) { return op(a.v / b.v); }
+inline Flop operator+(Flop a, double b) { return op(a.v + b); }
+inline Flop operator-(Flop a, double b) { return op(a.v - b); }
+inline Flop operator*(Flop a, double b) { return op(a.v * b); }
+inline Flop operator/(Flop a, double b) { return op(a.v / b); }
+inline Flop operator+(double a, Flop b) { return op(a + b.v); }
+inline Flop operator-(double a, Flop b) { return op(a - b.v); }
+inline Flop operator*(double a, Flop b) { return op(a * b.v); }
+inline Flop operator/(double a, Flop b) { return op(a / b.v); }
+inline Flop operator-(Flop a) { return Flop(-a.v); }
+inline Flop& operator+=(Flop& a, Flop b) { a = a + b; return a; }
+inline Flop& operator*=(Flop& a, Flop b) { a = a * b; return a; }
+inline bool operator<(Flop a, Flop b) { return a.v < b.v; }
+inline bool operator<(Flop a, double b) { return a.v < b; }
+inline bool operator<(double a, Flop b) { return a < b.v; }
+inline bool operator>(Flop a, double b) { return a.v > b; }
+inline bool operator<=(Flop a, double b) { return a.v <= b; }
+inline Flop sqrt(Flop a) { return op(std::sqrt(a.v)); }
+inline Flop sin(Flop a) { return op(std::sin(a.v)); }
+inline Flop cos(Flop a) { return op(std::cos(a.v)); }
+inline Flop pow(Flop a, double b) { return op(std::pow(a.v, b)); }
+inline Flop fabs(Flop a) { return Flop(std::fabs(a.v)); }
+inline bool finite_t(Flop x) { return std::isfinite(x.v); }
+
+// A ray state well inside the integration region (r = 12 M, equatorial
+// photon of the default camera's kind), for counting.
+void flop_state(double y[8]) {
+    const double s[8] = {0.0, 12.0, 1.3, 0.4, 0.998, -0.31, 2.1, -3.7};
+    for (int i = 0; i < 8; ++i) y[i] = s[i];
+}
+
+}  // namespace
+
 extern "C" {
+
+// Algorithmic FP64 operations of the reference's path, counted by
+// instantiating the restatement with Flop (out: per RHS evaluation; per DP5
+// attempt excluding its 6 RHS, accepted (incl. dense output and controller)
+// and rejected; per accepted step of the trace loop outside the attempt:
+// invariants, pseudo-Cartesian end point, the hmax clamp, the drift maxima and
+// the disc's sign test (render.cpp:130-131, 140-145, 150, 181-183)).
+void oracle_flop_counts(int64_t* out) {
+    double y0[8];
+    flop_state(y0);
+    Flop y[8], d[8];
+    for (int i = 0; i < 8; ++i) y[i] = y0[i];
+    const Flop a = 0.9;
+    g_flops = 0;
+    rhs_t<Flop>(a, y, d);
+    const long long per_rhs = g_flops;
+    auto attempt_ops = [&](double h0, bool* accepted) {
+        DP5T<Flop> st{a, Flop(1e-9), Flop(1e-11), {}, false, Flop(1e-4), 0};
+        rhs_t<Flop>(a, y, st.k1);
+        st.k1_ok = true;
+        Flop h = h0, yn[8];
+        SegT<Flop> seg;
+        g_flops = 0;
+        *accepted = st.attempt(y, h, yn, seg);
+        return g_flops - 6 * per_rhs;
+    };
+    bool acc_small = false, acc_big = true;
+    const long long att_acc = attempt_ops(1e-3, &acc_small);
+    const long long att_rej = attempt_ops(5.0, &acc_big);
+    // per accepted step outside the attempt
+    g_flops = 0;
+    const InvT<Flop> iv = invariants_t<Flop>(a, y);
+    Flop p1[3];
+    to_cartesian_t<Flop>(a, y[YR], y[YH], y[YP], p1);
+    const long long inv_ops = g_flops;
+    const long long loop_ops = 3      // hmax = 0.05 max(1, r - r_h) + 0.01
+                             + 2      // (Q - Q0) / (|Q0| + 1e-12): the drift of Carter's Q
+                             + 3;     // cos(theta_0), cos(theta_1), their product
+    (void)iv;
+    out[0] = per_rhs;
+    out[1] = acc_small ? att_acc : -1;
+    out[2] = acc_big ? -1 : att_rej;
+    out[3] = inv_ops + loop_ops;
+    out[4] = 6 * per_rhs + out[1] + out[3];   // one accepted step
+    out[5] = 6 * per_rhs + out[2];            // one rejected attempt
+}
+
 
 int oracle_render(const bhx_scene* s, const oracle_catalogue* cat, double* xyz, bhx_stats* stats,
                   int threads) {

@@ -76,6 +76,7 @@ int oracle_integrate(double a, double *y, double h0, double rtol, double atol, i
 double oracle_isco(double a, int sense);
 double oracle_page_thorne_W(double a, double r, double r_isco);
 double oracle_disc_temperature(const bhx_scene *sc, double r);
+void oracle_flop_counts(int64_t *out);
 void oracle_disc_intensity(const bhx_scene *sc, int64_t n, const uint8_t *cls, const double *r, const double *g,
                            double *out);
 void oracle_blackbody_xyz(double T, double *xyz);
