@@ -33,6 +33,8 @@ class Scene(C.Structure):
                                     "disc_enabled", "disc_sense", "stars_enabled", "star_count",
                                     "sun_enabled", "hs_enabled", "hs_sense", "ns_enabled",
                                     "ns_caps", "ns_two_caps", "jet_enabled", "threads")]
+        + [("jet_model", C.c_int32), ("jet_reserved", C.c_int32)]
+        + [(n, C.c_double) for n in ("jet_cone_angle", "jet_cone_r0", "jet_cone_r1")]
     )
 
     def copy(self):

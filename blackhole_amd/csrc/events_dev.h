@@ -78,6 +78,42 @@ __device__ inline double jet_boost(const EventParams& J, double a, double a2, do
     return pow_pos(1.0 / ku, 2.0 + J.alpha);
 }
 
+// Conical jet (model 1, no counterpart in the reference): comoving emissivity
+// profile at (r, cos theta) - (r0 / r)^2 inside two cones of half opening
+// cone_angle about the spin axis, smooth over the outer fifth of the opening
+// and at both ends (the end ramps of the reference's jets, jet.h:199-202).
+__device__ inline double cone_shape(const EventParams& J, double r, double cs) {
+    if (r <= J.cone_r0 || r >= J.cone_r1) return 0.0;
+    const double psi = acos(dmin(fabs(cs), 1.0));
+    const double ta = smooth01((J.cone_angle - psi) / (0.2 * J.cone_angle));
+    if (ta <= 0.0) return 0.0;
+    const double q = J.cone_r0 / r;
+    const double ramp = smooth01((r - J.cone_r0) / (0.5 * J.cone_r0)) *
+                        smooth01((J.cone_r1 - r) / (0.15 * J.cone_r1));
+    return q * q * ta * ramp;
+}
+
+// g^(2+alpha) of the conical jet's plasma, radial outflow u = Gamma (e_t +
+// beta e_r) on the ZAMO tetrad, seen along momentum p.  0 when g <= 0.
+__device__ inline double cone_boost(const EventParams& J, double a, double a2, double r, double th,
+                                    double pt, double pr, double pph) {
+    double sn, cs;
+    sincos_near(th, &sn, &cs);
+    if (fabs(sn) < 1e-9) sn = (sn < 0.0) ? -1e-9 : 1e-9;
+    const double sn2 = sn * sn;
+    const double S = r * r + a2 * cs * cs;
+    const double D = r * r - 2.0 * r + a2;
+    const double A = (r * r + a2) * (r * r + a2) - a2 * D * sn2;
+    const double lapse = sqrt(S * D / A);
+    const double omega = 2.0 * a * r / A;
+    const double u0 = J.cone_gamma / lapse;
+    const double u1 = J.cone_gamma * J.cone_beta * sqrt(D / S);
+    const double u3 = J.cone_gamma * omega / lapse;
+    const double ku = pt * u0 + pr * u1 + pph * u3;
+    if (!(ku > 1e-12)) return 0.0;
+    return pow_pos(1.0 / ku, 2.0 + J.alpha);
+}
+
 // bl_to_cartesian (geodesic.h:256-262).
 __device__ __forceinline__ void to_cart(double a2, double r, double th, double ph, double o[3]) {
     double sn, cs, sp, cp;

@@ -265,6 +265,12 @@ EventParams event_params(const bhx_scene& s) {
     E.gamma_zb = 1.0 + (s.jet_gamma_max - 1.0) * t;
     for (int i = 0; i < 3; ++i) E.sun_c[i] = s.sun_centre[i];
     E.sun_R = s.sun_radius;
+    E.model = s.jet_model;
+    E.cone_angle = s.jet_cone_angle;
+    E.cone_r0 = s.jet_cone_r0;
+    E.cone_r1 = s.jet_cone_r1;
+    E.cone_gamma = std::max(1.0, s.jet_gamma_max);
+    E.cone_beta = std::sqrt(std::max(0.0, 1.0 - 1.0 / (E.cone_gamma * E.cone_gamma)));
     return E;
 }
 

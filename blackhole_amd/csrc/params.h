@@ -11,6 +11,9 @@ struct EventParams {
     double z_base, z_top, R_base, k, gamma_max, z_sat, alpha, u_peak, u_wid, spine, spine_wid;
     double zeta, Omega_F, j_nu0, r_bound, gamma_zb;   // gamma_zb = Jet::gamma_at(z_base)
     double sun_c[3], sun_R;
+    // conical jet (model 1): half opening, radial extent, outflow speed and Lorentz factor
+    double cone_angle, cone_r0, cone_r1, cone_beta, cone_gamma;
+    int32_t model, pad;
 };
 
 // One scene of a batched launch: the tracer's per-scene constants (the

@@ -375,11 +375,14 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
                         const double ths = dense_comp(c1[2], c2[2], c3[2], c4[2], c5[2], sv);
                         double sn, cs;
                         sincos_near(ths, &sn, &cs);
-                        const double sh = jet_shape(J, rs * cs, sqrt(rs * rs + la2) * sn);
+                        const double sh = (J.model == 1) ? cone_shape(J, rs, cs)
+                                                         : jet_shape(J, rs * cs, sqrt(rs * rs + la2) * sn);
                         if (sh <= 0.0) continue;
-                        const double b = jet_boost(J, la, la2, rs, ths, pt,
-                                                   dense_comp(c1[4], c2[4], c3[4], c4[4], c5[4], sv),
-                                                   dense_comp(c1[5], c2[5], c3[5], c4[5], c5[5], sv), pph);
+                        const double prs = dense_comp(c1[4], c2[4], c3[4], c4[4], c5[4], sv);
+                        const double b = (J.model == 1)
+                            ? cone_boost(J, la, la2, rs, ths, pt, prs, pph)
+                            : jet_boost(J, la, la2, rs, ths, pt, prs,
+                                        dense_comp(c1[5], c2[5], c3[5], c4[5], c5[5], sv), pph);
                         if (b <= 0.0) continue;
                         sum += kGLw[qd] * sh * b;
                     }

@@ -17,7 +17,7 @@ SYMBOLS = (
     "bhx_create", "bhx_destroy", "bhx_last_error", "bhx_abi_version", "bhx_set_catalogue",
     "bhx_render", "bhx_trace_dev", "bhx_shade_dev", "bhx_auto_exposure_dev", "bhx_develop_dev",
     "bhx_blackbody_xyz_dev", "bhx_set_launch", "bhx_get_launch", "bhx_setup_jet",
-    "bhx_polytrope_star", "bhx_setup_neutron_star", "bhx_batch_create", "bhx_batch_destroy",
+    "bhx_polytrope_star", "bhx_setup_neutron_star", "bhx_setup_conical_jet", "bhx_batch_create", "bhx_batch_destroy",
     "bhx_batch_run", "bhx_batch_scene", "bhx_batch_copy_scene", "bhx_batch_stats",
     "bhx_batch_set_launch", "bhx_batch_get_launch", "bhx_batch_autotune", "bhx_adjoint_create",
     "bhx_adjoint_destroy", "bhx_adjoint_set_scene", "bhx_adjoint_forward", "bhx_adjoint_backward",
@@ -57,6 +57,7 @@ def lib():
     L.bhx_polytrope_star.argtypes = [C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]
     L.bhx_setup_neutron_star.argtypes = [C.POINTER(Scene), C.c_double, C.c_double]
+    L.bhx_setup_conical_jet.argtypes = [C.POINTER(Scene), C.c_double, C.c_double]
     L.bhx_batch_create.argtypes = [P, C.POINTER(Scene), C.c_int32, C.POINTER(P)]
     L.bhx_batch_destroy.argtypes = [P]
     L.bhx_batch_run.argtypes = [P, C.c_int32, P]

@@ -107,6 +107,22 @@ def setup_jet(s, phi_flux=50.0, eps_rad=0.02):
     return s
 
 
+def setup_conical_jet(s, half_angle_deg=5.0, r0=6.0, r1=500.0, gamma=10.0, alpha=0.7,
+                      phi_flux=50.0, eps_rad=0.02):
+    """Give the scene the build's conical jet (no reference counterpart): two
+    cones of half opening half_angle_deg about the spin axis from r0 to r1
+    (GM/c^2), radial outflow at Lorentz factor gamma, comoving emissivity
+    ~ (r0/r)^2 nu^-alpha, normalised like the reference's jets (the radiated
+    fraction eps_rad of the Blandford-Znajek power)."""
+    from . import native
+    s.jet_enabled, s.jet_model = 1, 1
+    s.jet_cone_angle, s.jet_cone_r0, s.jet_cone_r1 = math.radians(half_angle_deg), r0, r1
+    s.jet_gamma_max, s.jet_alpha = gamma, alpha
+    native.check(None, native.lib().bhx_setup_conical_jet(C.byref(s), float(phi_flux), float(eps_rad)),
+                 "bhx_setup_conical_jet")
+    return s
+
+
 def polytrope_star(K_si, gamma, M_kg):
     """(R_m, M_kg) of the stable star of mass M_kg for P = K rho^gamma
     (src/tov.h:205-218); BhxError when the equation of state cannot hold it."""
@@ -182,3 +198,22 @@ def disc_frame(width=512, height=512, spin=0.7, incl_deg=45.0, eddington=0.1, r_
                     rtol=rtol, atol=atol)
     s.r_escape = 3.0 * 500.0 + 100.0
     return s
+
+
+def quasar_frame(incl_deg, width=1024, height=1024, stars=True, half_extent=1050.0):
+    """One frame of the quasar sweep: a = 0.998, NT disc ISCO-1000M at 0.5
+    Eddington, the conical jet (5 deg half opening, Gamma 10, (r0/r)^2 out to
+    500 M), 250k-star sky; camera at 2500 M (outside the disc) framing
+    +-half_extent M (default the whole disc)."""
+    s = plain_scene(spin=0.998, incl_deg=incl_deg, r_cam=2500.0, width=width, height=height,
+                    fov=2.0 * math.atan(half_extent / 2500.0), disc_r_out=1000.0, eddington=0.5,
+                    stars=stars)
+    setup_conical_jet(s)
+    s.r_escape = escape_radius(s)
+    return s
+
+
+def quasar_sweep(n_frames=120, width=1024, height=1024, stars=True):
+    """Inclination swept linearly 0 -> 90 deg over n_frames (clamped at 0.5
+    deg like the CLI, src/main.cpp:810)."""
+    return [quasar_frame(90.0 * i / (n_frames - 1), width, height, stars) for i in range(n_frames)]

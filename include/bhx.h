@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define BHX_ABI_VERSION 1
+#define BHX_ABI_VERSION 2
 
 /* Error codes. */
 #define BHX_OK 0
@@ -96,6 +96,14 @@ typedef struct bhx_scene {
     int32_t disc_enabled, disc_sense, stars_enabled, star_count;
     int32_t sun_enabled, hs_enabled, hs_sense, ns_enabled;
     int32_t ns_caps, ns_two_caps, jet_enabled, threads;
+    /* Jet model: 0 = the reference's Jet (src/jet.h), 1 = a conical jet that
+     * the reference does not have: two cones about the spin axis of half
+     * opening jet_cone_angle [rad] from jet_cone_r0 to jet_cone_r1 [GM/c^2],
+     * radial outflow at Lorentz factor jet_gamma_max in the ZAMO frame,
+     * comoving emissivity jet_j_nu0 (r0 / r)^2 nu^-jet_alpha with smooth
+     * edges; integrated along the ray like the reference's jets. */
+    int32_t jet_model, jet_reserved;
+    double jet_cone_angle, jet_cone_r0, jet_cone_r1;
 } bhx_scene;
 
 /* RenderStats (src/render.h:55-69) plus the counters the reference lacks. */
@@ -303,6 +311,12 @@ int bhx_adjoint_info(bhx_adjoint *adj, int64_t *ck_bytes, int64_t *disc_rays, in
  * jet_j_nu0, jet_r_bound.  Clears jet_enabled when the scene has no disc, is a
  * neutron-star scene, or the Blandford-Znajek power is zero (a = 0). */
 int bhx_setup_jet(bhx_scene *scene, double phi_flux, double eps_rad);
+
+/* The same normalisation for the conical jet (jet_model = 1): the radiated
+ * fraction eps_rad of the Blandford-Znajek power spread over both cones'
+ * emissivity profile; writes jet_a, jet_Omega_F, jet_j_nu0 and jet_r_bound
+ * (1.05 jet_cone_r1).  No counterpart in the reference. */
+int bhx_setup_conical_jet(bhx_scene *scene, double phi_flux, double eps_rad);
 
 /* Stable-branch star of gravitational mass M_kg for the polytrope
  * P = K_si rho^Gamma (SI units): ns::star_of_mass (src/tov.h:205-218) over

@@ -666,6 +666,30 @@ struct Disc {  // disc.h:66-151
 struct JetModel {
     double z_base, z_top, R_base, k, gamma_max, z_sat, alpha, u_peak, u_wid, spine, spine_wid;
     double zeta, Omega_F, j_nu0, r_bound;
+    // conical jet (bhx_scene.jet_model = 1), the build's own model: no
+    // reference counterpart; the same formulas as the device's cone_shape /
+    // cone_boost, written with the C library's functions
+    int model = 0;
+    double cone_angle = 0.0, cone_r0 = 0.0, cone_r1 = 0.0;
+    double cone_shape(double r, double cs) const {
+        if (r <= cone_r0 || r >= cone_r1) return 0.0;
+        const double psi = std::acos(std::min(std::fabs(cs), 1.0));
+        const double ta = smooth01((cone_angle - psi) / (0.2 * cone_angle));
+        if (ta <= 0.0) return 0.0;
+        const double q = cone_r0 / r;
+        const double ramp = smooth01((r - cone_r0) / (0.5 * cone_r0)) * smooth01((cone_r1 - r) / (0.15 * cone_r1));
+        return q * q * ta * ramp;
+    }
+    void cone_velocity(const Pt& p, double u[4]) const {
+        const double G = std::max(1.0, gamma_max);
+        const double beta = std::sqrt(std::max(0.0, 1.0 - 1.0 / (G * G)));
+        const double lapse = std::sqrt(p.S * p.D / p.A);
+        const double omega = 2.0 * p.a * p.r / p.A;
+        u[0] = G / lapse;
+        u[1] = G * beta * std::sqrt(p.D / p.S);
+        u[2] = 0.0;
+        u[3] = G * omega / lapse;
+    }
     double radius(double az) const { return R_base * std::pow(std::max(az, 1e-6) / z_base, k); }
     double gamma_at(double az) const {
         const double t = std::min(1.0, std::pow(std::max(az, 1e-6) / z_sat, k));
@@ -787,6 +811,10 @@ struct Scene {
         jet = JetModel{c.jet_z_base, c.jet_z_top, c.jet_R_base, c.jet_k, c.jet_gamma_max,
                        c.jet_z_sat, c.jet_alpha, c.jet_u_peak, c.jet_u_wid, c.jet_spine,
                        c.jet_spine_wid, c.jet_zeta, c.jet_Omega_F, c.jet_j_nu0, c.jet_r_bound};
+        jet.model = c.jet_model;
+        jet.cone_angle = c.jet_cone_angle;
+        jet.cone_r0 = c.jet_cone_r0;
+        jet.cone_r1 = c.jet_cone_r1;
     }
     ~Scene() { delete disc; }
     Scene(const Scene&) = delete;
@@ -897,8 +925,26 @@ void trace_state(const Scene& sc, const double y0[8], RayResult& out) {
     double p0[3];
     to_cartesian(a, y[YR], y[YH], y[YP], p0);
 
+    // the conical jet (the build's own model) has a sampler of its own so
+    // that the reference's path stays the reference's code
+    auto cone_sample = [&](const Seg& seg, double h_step, double s_end) {
+        double sum = 0.0;
+        for (int q = 0; q < 4; ++q) {
+            double ys[8];
+            seg.at(kGLx[q] * s_end, ys);
+            const double sh = sc.jet.cone_shape(ys[YR], std::cos(ys[YH]));
+            if (sh <= 0.0) continue;
+            double u[4];
+            sc.jet.cone_velocity(point(a, ys[YR], ys[YH]), u);
+            const double gf = redshift(ys, u);
+            if (gf <= 0.0) continue;
+            sum += kGLw[q] * sh * sc.jet.boost(gf);
+        }
+        out.jet += c.jet_j_nu0 * sum * h_step * s_end;
+    };
     auto jet_sample = [&](const Seg& seg, double h_step, double s_end) {
         if (!c.jet_enabled || s_end <= 0.0) return;
+        if (sc.jet.model == 1) { cone_sample(seg, h_step, s_end); return; }
         double sum = 0.0;
         for (int q = 0; q < 4; ++q) {
             double ys[8];
