@@ -5,11 +5,11 @@
 //   k_adj_forward   one ray per lane; a lane whose ray ends takes the next ray
 //                   index from a global counter in the same iteration, so every
 //                   iteration is one DP5 attempt on every lane.  Writes the
-//                   class, hit state, accepted-step count and bolometric
-//                   intensity of each sample, and a checkpoint of the
+//                   class, hit state, accepted-step count and intensity
+//                   (disc plus jet) of each sample, and a checkpoint of the
 //                   integrator state every K accepted steps (HBM slab of
 //                   cap = max_steps / K + 1 records of 8 doubles per ray).
-//   k_adj_backward  one disc-hit ray per lane, rays sorted by step count so the
+//   k_adj_backward  one lit ray (I > 0) per lane, sorted by step count so the
 //                   lanes of a wave replay and sweep segments of equal length
 //                   together; each lane's K-step segment buffer is a strided
 //                   HBM scratch.  Per-block partial gradients, summed on the
@@ -38,13 +38,13 @@ struct AdjArgs {
     uint8_t* cls;
     double* y;                // 8 x n_rays SoA (layout of bhx_rays.y)
     int32_t* steps;
-    double* I;                // n_rays bolometric intensity
-    int32_t* key;             // sort key: accepted steps of a disc hit, else 0
-    unsigned int* n_disc;     // disc hits of the forward
+    double* I;                // n_rays intensity: disc sigma (g T)^4 / pi + jet kappa J
+    int32_t* key;             // sort key: accepted steps of a lit ray, else 0
+    unsigned int* n_lit;      // lit rays (I > 0) of the forward
     // backward
     const double* w;          // dL/dI per sample
-    const int32_t* order;     // rays by key, descending: the disc hits first
-    double* seg;              // K * 5 doubles per lane, strided by seg_stride
+    const int32_t* order;     // rays by key, descending: the lit rays first
+    double* seg;              // K * kSeg doubles per lane, strided by seg_stride
     int64_t seg_stride;
     double* partial;          // 4 per block
     unsigned int* mismatch;
@@ -83,9 +83,10 @@ __global__ void __launch_bounds__(256) k_adj_forward(const AdjConst* __restrict_
             P.y[ray] = yo[0]; P.y[N + ray] = yo[1]; P.y[2 * N + ray] = yo[2]; P.y[3 * N + ray] = yo[3];
             P.y[4 * N + ray] = s.pt; P.y[5 * N + ray] = yo[4]; P.y[6 * N + ray] = yo[5]; P.y[7 * N + ray] = s.pph;
             P.steps[ray] = s.nsteps;
-            P.I[ray] = (term == BHX_DISC) ? disc_intensity(F, yo[1], s.pt, s.pph) : 0.0;
-            P.key[ray] = (term == BHX_DISC) ? min(s.nsteps, 32767) : 0;   // 15-bit key, >= 1 for a disc hit
-            if (term == BHX_DISC) atomicAdd(P.n_disc, 1u);
+            const double I = ((term == BHX_DISC) ? disc_intensity(F, yo[1], s.pt, s.pph) : 0.0) + F.kappa * s.jet;
+            P.I[ray] = I;
+            P.key[ray] = (I > 0.0) ? max(1, min(s.nsteps, 32767)) : 0;   // 15-bit key, >= 1 for a lit ray
+            if (I > 0.0) atomicAdd(P.n_lit, 1u);
             live = false;
         }
     }
@@ -105,12 +106,12 @@ __global__ void __launch_bounds__(256) k_adj_backward(const AdjConst* __restrict
     const int64_t slab = static_cast<int64_t>(F.cap) * kCk;
     double g[4] = {0.0, 0.0, 0.0, 0.0};
     unsigned int mism = 0;
-    const int64_t n_order = *P.n_disc;
+    const int64_t n_order = *P.n_lit;
     for (int64_t i = lane; i < n_order; i += nl) {
         const int64_t ray = P.order[i];
         const double w = P.w[ray];
         if (w == 0.0) continue;
-        mism += bwd_ray(F, G, ray, P.steps[ray], w, P.ck + ray * slab, P.seg + lane, P.seg_stride, g);
+        mism += bwd_ray(F, G, ray, P.steps[ray], P.cls[ray], w, P.ck + ray * slab, P.seg + lane, P.seg_stride, g);
     }
     // block sums in a fixed order
 #pragma unroll
@@ -156,7 +157,9 @@ int adj_set_scene(bhx_adjoint* A, const bhx_scene* s) {
         return bhx_fail(ctx, BHX_E_INVALID, "max_steps differs from the adjoint's");
     AdjConst c;
     if (!make_frame(*s, A->K, c.F, c.G))
-        return bhx_fail(ctx, BHX_E_INVALID, "only disc scenes without neutron star, jets, Sun or hot spot are differentiable");
+        return bhx_fail(ctx, BHX_E_INVALID,
+                        "only disc scenes (optionally with the conical jet) without neutron star, Sun or hot spot are "
+                        "differentiable");
     if (!(s->rtol > 0.0) || !(s->atol > 0.0)) return bhx_fail(ctx, BHX_E_INVALID, "rtol/atol must be > 0");
     if (!(s->eddington > 0.0)) return bhx_fail(ctx, BHX_E_INVALID, "eddington must be > 0");
     // the constants are read by kernels of earlier calls: replace them in order
@@ -207,7 +210,7 @@ int bhx_adjoint_create(bhx_context* ctx, const bhx_scene* s, int32_t K, bhx_adjo
         A->steps.ensure(sizeof(int32_t) * n) != hipSuccess || A->I.ensure(sizeof(double) * n) != hipSuccess ||
         A->order.ensure(sizeof(int32_t) * n) != hipSuccess || A->key.ensure(sizeof(int32_t) * n) != hipSuccess ||
         A->key_sorted.ensure(sizeof(int32_t) * n) != hipSuccess || A->iota.ensure(sizeof(int32_t) * n) != hipSuccess ||
-        A->seg.ensure(sizeof(double) * 5 * K * lanes) != hipSuccess ||
+        A->seg.ensure(sizeof(double) * kSeg * K * lanes) != hipSuccess ||
         A->partial.ensure(sizeof(double) * 4 * A->grid_b) != hipSuccess ||
         A->counters.ensure(64) != hipSuccess)
         return bail(bhx_fail(ctx, BHX_E_NOMEM, "adjoint buffers"));
@@ -265,14 +268,14 @@ int bhx_adjoint_forward(bhx_adjoint* A, double* I, void* stream) {
     P.steps = static_cast<int32_t*>(A->steps.p);
     P.I = I ? I : static_cast<double*>(A->I.p);
     P.key = static_cast<int32_t*>(A->key.p);
-    P.n_disc = reinterpret_cast<unsigned int*>(static_cast<char*>(A->counters.p) + 8);
-    // counters: [0] queue, [8] disc hits, [32] replay mismatches
+    P.n_lit = reinterpret_cast<unsigned int*>(static_cast<char*>(A->counters.p) + 8);
+    // counters: [0] queue, [8] lit rays, [32] replay mismatches
     HIPCHK(launch_zero_u64(static_cast<unsigned long long*>(A->counters.p), 8, st));
     HIPCHK(hipEventRecord(A->ev[0], st));
     hipLaunchKernelGGL(k_adj_forward, dim3(A->grid_f), dim3(256), 0, st,
                        static_cast<const AdjConst*>(A->cst.p), P);
     HIPCHK(hipGetLastError());
-    // the backward's ray order: disc hits first, most accepted steps first
+    // the backward's ray order: lit rays first, most accepted steps first
     // (a stable radix sort on the 15-bit step count: deterministic)
     size_t tmp = A->sort_tmp.bytes;
     HIPCHK(hipcub::DeviceRadixSort::SortPairsDescending(A->sort_tmp.p, tmp, static_cast<const int32_t*>(A->key.p),
@@ -292,6 +295,7 @@ int bhx_adjoint_backward(bhx_adjoint* A, const double* w, double* grad, void* st
     AdjArgs P{};
     P.n_rays = A->n_rays;
     P.ck = static_cast<double*>(A->ck.p);
+    P.cls = static_cast<uint8_t*>(A->cls.p);
     P.steps = static_cast<int32_t*>(A->steps.p);
     P.w = w;
     P.order = static_cast<const int32_t*>(A->order.p);
@@ -299,7 +303,7 @@ int bhx_adjoint_backward(bhx_adjoint* A, const double* w, double* grad, void* st
     P.seg_stride = static_cast<int64_t>(A->grid_b) * 256;
     P.partial = static_cast<double*>(A->partial.p);
     P.mismatch = reinterpret_cast<unsigned int*>(static_cast<char*>(A->counters.p) + 32);
-    P.n_disc = reinterpret_cast<unsigned int*>(static_cast<char*>(A->counters.p) + 8);
+    P.n_lit = reinterpret_cast<unsigned int*>(static_cast<char*>(A->counters.p) + 8);
     HIPCHK(launch_zero_u32(P.mismatch, 1, st));
     HIPCHK(hipEventRecord(A->ev[2], st));
     hipLaunchKernelGGL(k_adj_backward, dim3(A->grid_b), dim3(256), 0, st,
@@ -342,7 +346,7 @@ int bhx_adjoint_copy_rays(bhx_adjoint* A, uint8_t* cls, double* y, int32_t* step
     return BHX_OK;
 }
 
-int bhx_adjoint_info(bhx_adjoint* A, int64_t* ck_bytes, int64_t* disc_rays, int64_t* mismatches,
+int bhx_adjoint_info(bhx_adjoint* A, int64_t* ck_bytes, int64_t* lit_rays, int64_t* mismatches,
                      double* fwd_ms, double* bwd_ms) {
     if (!A) return BHX_E_INVALID;
     bhx_context* ctx = A->ctx;
@@ -355,7 +359,7 @@ int bhx_adjoint_info(bhx_adjoint* A, int64_t* ck_bytes, int64_t* disc_rays, int6
     unsigned int nd = 0;
     HIPCHK(hipMemcpy(&nd, static_cast<char*>(A->counters.p) + 8, sizeof nd, hipMemcpyDeviceToHost));
     if (ck_bytes) *ck_bytes = static_cast<int64_t>(A->ck.bytes);
-    if (disc_rays) *disc_rays = nd;
+    if (lit_rays) *lit_rays = nd;
     if (mismatches) *mismatches = A->mismatches;
     if (fwd_ms) *fwd_ms = A->fwd_ms;
     if (bwd_ms) *bwd_ms = A->bwd_ms;

@@ -48,11 +48,18 @@ struct Frame {
     double pt_x0, pt_c[3], pt_x[3], d_x0, d_c[3], d_x[3];
     // d(e00 e03 e11 e22 e33 gl_tt gl_tp gl_rr gl_hh gl_pp)/d(a, r_cam, theta_cam)
     double dcam[10][3];
+    // conical jet (bhx_scene.jet_model = 1), integrated along the ray: its
+    // intensity kappa * J joins the disc's (kappa = nu0^alpha x band integral,
+    // J the specific intensity at nu0); d ln j_nu0 / d spin from the
+    // Blandford-Znajek power and the disc efficiency
+    int32_t has_jet, pad_;
+    double j0, alpha, kappa, dlnj0_da, r_bound;
+    double cone_angle, cone_r0, cone_r1, cone_beta, cone_gamma, cone_gate;
 };
 
 // Integrator state of one ray.
 struct Ray {
-    double y[6], k1[6], pt, pph, h, pw, c_prev;
+    double y[6], k1[6], pt, pph, h, pw, c_prev, jet;
     int nsteps, tries;
 };
 
@@ -213,6 +220,101 @@ __host__ __device__ inline void disc_intensity_grad(const Frame& F, double r, do
     for (int k = 0; k < 4; ++k) dI[k] = I.d[k];
 }
 
+// ---- conical jet -------------------------------------------------------------
+
+template <class T>
+__host__ __device__ inline T smooth_c2_t(const T& t) {
+    if (value(t) <= 0.0) return T(0.0);
+    if (value(t) >= 1.0) return T(1.0);
+    return t * t * t * (10.0 + t * (6.0 * t - 15.0));
+}
+template <class T>
+__host__ __device__ inline T abs_t(const T& x) { return (value(x) < 0.0) ? -x : x; }
+
+// Integrand of the conical jet at dense-output state (r, theta, p_r) of a
+// ray with constants (a, p_t, p_phi): comoving profile (r0/r)^2 x taper x
+// ramps, times g^(2+alpha) of the radial Gamma outflow on the ZAMO tetrad -
+// the device's cone_shape x cone_boost, generic in the scalar type so that
+// dual numbers give its gradient.
+template <class T>
+__host__ __device__ inline T cone_F(const Frame& F, const T& r, const T& th, const T& pr, const T& a, const T& pt,
+                                    const T& pph) {
+    if (value(r) <= F.cone_r0 || value(r) >= F.cone_r1) return T(0.0);
+    T sn = sin(th);
+    const T cs = cos(th);
+    T acs = abs_t(cs);
+    if (value(acs) > 1.0) acs = T(1.0);
+    const T psi = acos(acs);
+    const T ta = smooth_c2_t((F.cone_angle - psi) / (0.2 * F.cone_angle));
+    if (!(value(ta) > 0.0)) return T(0.0);
+    const T q = F.cone_r0 / r;
+    const T ramp = smooth_c2_t((r - F.cone_r0) / (0.5 * F.cone_r0)) * smooth_c2_t((F.cone_r1 - r) / (0.15 * F.cone_r1));
+    if (fabs(value(sn)) < 1e-9) sn = T((value(sn) < 0.0) ? -1e-9 : 1e-9);
+    const T a2 = a * a;
+    const T S = r * r + a2 * cs * cs;
+    const T D = r * r - 2.0 * r + a2;
+    const T A = (r * r + a2) * (r * r + a2) - a2 * D * sn * sn;
+    const T lapse = sqrt(S * D / A);
+    const T omega = 2.0 * a * r / A;
+    const T ku = pt * (F.cone_gamma / lapse) + pr * (F.cone_gamma * F.cone_beta * sqrt(D / S)) +
+                 pph * (F.cone_gamma * omega / lapse);
+    if (!(value(ku) > 1e-12)) return T(0.0);
+    return q * q * ta * ramp * pow(1.0 / ku, 2.0 + F.alpha);
+}
+
+// pseudo-Cartesian point (geodesic.h:256-262), the segment's distance to
+// the origin (render.cpp:55-62) and the conical jet's sampling gate: the
+// jet's cull, as the tracer does it
+__host__ __device__ inline void cart_hd(double a2, double r, double th, double ph, double o[3]) {
+    double sn, cs, sp, cp;
+    sincos_near(th, &sn, &cs);
+    sincos_near(ph, &sp, &cp);
+    const double rr = sqrt(r * r + a2);
+    o[0] = rr * sn * cp;
+    o[1] = rr * sn * sp;
+    o[2] = r * cs;
+}
+__host__ __device__ inline bool near_jet(const Frame& F, const double y[6], const double yn[6]) {
+    if (!F.has_jet) return false;
+    double p0[3], p1[3];
+    cart_hd(F.a2, y[1], y[2], y[3], p0);
+    cart_hd(F.a2, yn[1], yn[2], yn[3], p1);
+    const double d0 = p1[0] - p0[0], d1 = p1[1] - p0[1], d2 = p1[2] - p0[2];
+    const double dd = d0 * d0 + d1 * d1 + d2 * d2;
+    double t = 0.0;
+    if (dd > 0.0) t = dclamp(-(p0[0] * d0 + p0[1] * d1 + p0[2] * d2) / dd, 0.0, 1.0);
+    const double q0 = p0[0] + t * d0, q1 = p0[1] + t * d1, q2 = p0[2] + t * d2;
+    if (!(sqrt(q0 * q0 + q1 * q1 + q2 * q2) < F.r_bound)) return false;
+    // the sampling gate of the conical jet (z = r cos theta at either end)
+    return fabs(p0[2]) > F.cone_gate * y[1] || fabs(p1[2]) > F.cone_gate * yn[1];
+}
+
+// Four-point Gauss-Legendre nodes and weights on [0, 1] (render.cpp:47-50).
+__host__ __device__ inline double gl_x(int q) {
+    return q == 0 ? 0.06943184420297371 : q == 1 ? 0.33000947820757187 : q == 2 ? 0.66999052179242813
+                                                                              : 0.93056815579702629;
+}
+__host__ __device__ inline double gl_w(int q) { return (q == 0 || q == 3) ? 0.17392742256872693 : 0.32607257743127307; }
+
+// The jet's contribution of one step over the fraction [0, s_end] of its
+// dense output cf (components r, theta, p_r, p_theta): kConeSub
+// sub-intervals of 4-point Gauss-Legendre, j0 h s_end sum w_q F / kConeSub
+// (the tracer's sampling of the conical jet).
+__host__ __device__ inline double jet_step(const Frame& F, const double cf[4][5], double h, double s_end,
+                                           double pt, double pph) {
+    double sum = 0.0;
+    for (int k = 0; k < kConeSub; ++k)
+    for (int q = 0; q < 4; ++q) {
+        const double u = (k + gl_x(q)) / kConeSub;
+        const double sv = u * s_end;
+        const double r = dense_comp(cf[0][0], cf[0][1], cf[0][2], cf[0][3], cf[0][4], sv);
+        const double th = dense_comp(cf[1][0], cf[1][1], cf[1][2], cf[1][3], cf[1][4], sv);
+        const double pr = dense_comp(cf[2][0], cf[2][1], cf[2][2], cf[2][3], cf[2][4], sv);
+        sum += gl_w(q) * cone_F<double>(F, r, th, pr, F.a, pt, pph);
+    }
+    return F.j0 * sum * h * s_end / kConeSub;
+}
+
 // ---- reverse mode of the right-hand side -----------------------------------
 
 // Vector-Jacobian product of the (r, theta, p_r, p_theta) rates of kerr_rhs
@@ -336,11 +438,9 @@ __host__ __device__ inline void step_vjp(const RayK& K, const double x[4], doubl
     }
 }
 
-// Reverse of the terminal step: r_hit = r(s*) on the dense output of the step
-// from x with size h, where theta(s*) = pi/2.  Seeds the adjoint jr of r_hit;
-// adds the adjoint of x to xb and of (a, p_t, p_phi) to cb.
-__host__ __device__ inline void terminal_vjp(const RayK& K, const double x[4], double h, double s, double jr,
-                                             double xb[4], double cb[3]) {
+// Dense-output coefficients (c1..c5, Hairer's form, geodesic.h:177-185) of
+// the four components of the frozen step from x with size h.
+__host__ __device__ inline void step_dense(const RayK& K, const double x[4], double h, double cf[4][5]) {
     const double A[6][5] = {{0, 0, 0, 0, 0},
                             {A21, 0, 0, 0, 0},
                             {A31, A32, 0, 0, 0},
@@ -364,10 +464,7 @@ __host__ __device__ inline void terminal_vjp(const RayK& K, const double x[4], d
         yn[c] = x[c] + h * acc;
     }
     rhs_x(K, yn, k[6]);
-    // dense coefficients of r (0) and theta (1), their s-derivatives at s*
-    double dv[2];
-    double cf[2][5];
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < 4; ++c) {
         cf[c][0] = x[c];
         cf[c][1] = yn[c] - x[c];
         cf[c][2] = h * k[0][c] - cf[c][1];
@@ -375,25 +472,86 @@ __host__ __device__ inline void terminal_vjp(const RayK& K, const double x[4], d
         double acc = 0.0;
         for (int j = 0; j < 7; ++j) acc += Dd[j] * k[j][c];
         cf[c][4] = h * acc;
-        double v;
-        dense_vd(cf[c][0], cf[c][1], cf[c][2], cf[c][3], cf[c][4], s, v, dv[c]);
     }
-    // implicit function theorem: d r_hit = d r(s*) - r'(s*) / theta'(s*) d theta(s*)
-    const double yb[2] = {jr, -jr * dv[0] / dv[1]};
+}
+
+// A seed Yb on the dense output at s, as adjoints of the coefficients.
+__host__ __device__ inline void dense_seed(double s, const double Yb[4], double cbar[5][4]) {
     const double s1 = 1.0 - s;
     const double wts[5] = {1.0, s, s * s1, s * s * s1, s * s * s1 * s1};
-    double kb[7][4] = {};
-    double ynb[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int c = 0; c < 2; ++c) {
-        double cbr[5];
-        for (int m = 0; m < 5; ++m) cbr[m] = wts[m] * yb[c];
-        for (int j = 0; j < 7; ++j) kb[j][c] += h * Dd[j] * cbr[4];    // c5 = h sum d_j k_j
-        cbr[1] += cbr[3]; kb[6][c] += -h * cbr[3]; cbr[2] += -cbr[3];  // c4 = c2 - h k7 - c3
-        kb[0][c] += h * cbr[2]; cbr[1] += -cbr[2];                     // c3 = h k1 - c2
-        ynb[c] += cbr[1]; xb[c] += -cbr[1];                            // c2 = yn - x
-        xb[c] += cbr[0];                                               // c1 = x
+    for (int m = 0; m < 5; ++m)
+        for (int c = 0; c < 4; ++c) cbar[m][c] += wts[m] * Yb[c];
+}
+
+// Reverse of the coefficients' definitions: adjoints of c1..c5 become
+// adjoints of x, of yn and of the stage slopes k1..k7.
+__host__ __device__ inline void dense_coef_vjp(double h, const double cbar[5][4], double xb[4], double ynb[4],
+                                               double kb[7][4]) {
+    const double Dd[7] = {D1, 0.0, D3, D4, D5, D6, D7};
+    for (int c = 0; c < 4; ++c) {
+        double c2b = cbar[1][c], c3b = cbar[2][c];
+        const double c4b = cbar[3][c], c5b = cbar[4][c];
+        for (int j = 0; j < 7; ++j) kb[j][c] += h * Dd[j] * c5b;      // c5 = h sum d_j k_j
+        c2b += c4b; kb[6][c] += -h * c4b; c3b += -c4b;                 // c4 = c2 - h k7 - c3
+        kb[0][c] += h * c3b; c2b += -c3b;                              // c3 = h k1 - c2
+        ynb[c] += c2b; xb[c] += -c2b;                                  // c2 = yn - x
+        xb[c] += cbar[0][c];                                           // c1 = x
     }
+}
+
+// Adjoint seeds of wk * J_step, J_step = j0 h s_end sum_q w_q F(Y(x_q s_end)),
+// on the step's dense output: node seeds into cbar, constants (a, p_t, p_phi)
+// into cb.  Returns wk dJ_step/ds_end (for an event-truncated step).
+__host__ __device__ inline double jet_seeds(const Frame& F, const double cf[4][5], double h, double s_end,
+                                            double wk, double pt, double pph, double cbar[5][4], double cb[3]) {
+    using D6 = Dual<6>;
+    const double coef = wk * F.j0 * h;
+    double dJds = 0.0;
+    for (int k = 0; k < kConeSub; ++k)
+    for (int q = 0; q < 4; ++q) {
+        const double u = (k + gl_x(q)) / kConeSub;
+        const double sv = u * s_end;
+        double Y[4], Yp[4];
+        for (int c = 0; c < 4; ++c) dense_vd(cf[c][0], cf[c][1], cf[c][2], cf[c][3], cf[c][4], sv, Y[c], Yp[c]);
+        const D6 f = cone_F<D6>(F, D6::var(Y[0], 0), D6::var(Y[1], 1), D6::var(Y[2], 2), D6::var(F.a, 3),
+                                D6::var(pt, 4), D6::var(pph, 5));
+        if (f.v == 0.0) continue;
+        const double wq = coef * s_end * gl_w(q) / kConeSub;
+        const double Yb[4] = {wq * f.d[0], wq * f.d[1], wq * f.d[2], 0.0};
+        dense_seed(sv, Yb, cbar);
+        cb[0] += wq * f.d[3];
+        cb[1] += wq * f.d[4];
+        cb[2] += wq * f.d[5];
+        dJds += gl_w(q) / kConeSub * (f.v + s_end * u * (f.d[0] * Yp[0] + f.d[1] * Yp[1] + f.d[2] * Yp[2]));
+    }
+    return coef * dJds;
+}
+
+// Reverse of the terminal (disc-crossing) step from x with size h and
+// crossing fraction se: the seed Or on r(s*), the jet's sample over [0, s*]
+// with weight wk (when jet), and s* itself, fixed by theta(s*) = pi/2 on the
+// dense output (implicit-function theorem, ds* = -d theta(s*) / theta'(s*)).
+// Adds the adjoints of x to xb and of (a, p_t, p_phi) to cb; returns the
+// jet's sample J_step (0 without jet).
+__host__ __device__ inline double terminal_vjp(const Frame& F, const RayK& K, const double x[4], double h,
+                                               double se, double Or, bool jet, double wk, double xb[4],
+                                               double cb[3]) {
+    double cf[4][5], cbar[5][4] = {};
+    step_dense(K, x, h, cf);
+    double Os = 0.0, Js = 0.0;
+    if (jet) {
+        if (wk != 0.0) Os = jet_seeds(F, cf, h, se, wk, K.pt, K.pph, cbar, cb);
+        Js = jet_step(F, cf, h, se, K.pt, K.pph);
+    }
+    double rv, rd, tv, td;
+    dense_vd(cf[0][0], cf[0][1], cf[0][2], cf[0][3], cf[0][4], se, rv, rd);
+    dense_vd(cf[1][0], cf[1][1], cf[1][2], cf[1][3], cf[1][4], se, tv, td);
+    const double Yb[4] = {Or, -(Or * rd + Os) / td, 0.0, 0.0};
+    dense_seed(se, Yb, cbar);
+    double ynb[4] = {0.0, 0.0, 0.0, 0.0}, kb[7][4] = {};
+    dense_coef_vjp(h, cbar, xb, ynb, kb);
     step_vjp(K, x, h, ynb, kb, true, xb, cb);
+    return Js;
 }
 
 // ---- one ray, forward ----------------------------------------------------
@@ -425,14 +583,25 @@ __host__ __device__ inline void fwd_init(const Frame& F, const RaygenArgs& G, in
     s.c_prev = f.cs;
     s.h = F.h0;
     s.pw = F.pw_init;
+    s.jet = 0.0;
     s.nsteps = 0;
     s.tries = 0;
     write_ck(ck, s);
 }
 
+// Dense-output coefficients of the components (r, theta, p_r, p_theta).
+__host__ __device__ inline void dense4(const Ray& s, const Att& A, double cf[4][5]) {
+    dense_coeffs(s, A, 1, cf[0]);
+    dense_coeffs(s, A, 2, cf[1]);
+    dense_coeffs(s, A, 4, cf[2]);
+    dense_coeffs(s, A, 5, cf[3]);
+}
+
 // One attempt of the ray in s.  Returns its class once it ends (else -1),
-// with the end state in yo and, for a disc hit, the crossing fraction.
-// Checkpoints go to ck (this ray's slots) every K accepted steps.
+// with the end state in yo and, for a disc hit, the crossing fraction; the
+// jet integral accumulates in s.jet (render.cpp:107-122, 150-152: sampled on
+// every accepted step near the jet, over the part of the step before the
+// event that ends the ray).  Checkpoints go to ck every K accepted steps.
 __host__ __device__ inline int fwd_step(const Frame& F, Ray& s, double* ck, double yo[6], double* s_hit) {
     clamp_h(F, s);
     const RayK K = ray_k(F.a, F.a2, s.pt, s.pph);
@@ -444,6 +613,7 @@ __host__ __device__ inline int fwd_step(const Frame& F, Ray& s, double* ck, doub
         ++s.nsteps;
         s.tries = 0;
         int term = -1;
+        double s_jet = 1.0;
         const bool horizon = A.yn[1] <= F.r_stop;                    // render.cpp:172-177
         const bool disc_cross = !horizon && s.c_prev * A.g7.cs < 0.0;  // render.cpp:180-199
         if (horizon) {
@@ -463,8 +633,14 @@ __host__ __device__ inline int fwd_step(const Frame& F, Ray& s, double* ck, doub
                     yo[c] = dense_comp(cc[0], cc[1], cc[2], cc[3], cc[4], se);
                 }
                 *s_hit = se;
+                s_jet = se;
                 term = BHX_DISC;
             }
+        }
+        if (near_jet(F, s.y, A.yn)) {
+            double cf[4][5];
+            dense4(s, A, cf);
+            s.jet += jet_step(F, cf, s.h, s_jet, s.pt, s.pph);
         }
         if (term < 0 && A.yn[1] > F.r_escape) {                       // render.cpp:247-251
             for (int c = 0; c < 6; ++c) yo[c] = A.yn[c];
@@ -491,10 +667,12 @@ __host__ __device__ inline int fwd_step(const Frame& F, Ray& s, double* ck, doub
 
 // ---- one ray, backward ---------------------------------------------------
 
+constexpr int kSeg = 6;   // per stored step: r, theta, p_r, p_theta, h, jet-cull flag
+
 // Replay from the checkpoint at ck (the state after n_from accepted steps)
-// until n_to accepted steps: each accepted step's (r, theta, p_r, p_theta, h)
-// goes to seg[(i * 5 + c) * stride].  Same arithmetic as fwd_step, so the
-// same decisions and the same states.
+// until n_to accepted steps: each accepted step's (r, theta, p_r, p_theta, h,
+// near-jet flag) goes to seg[(i * kSeg + c) * stride].  Same arithmetic as
+// fwd_step, so the same decisions and the same states.
 __host__ __device__ inline void replay(const Frame& F, Ray& s, const double* ck, int n_from, int n_to, double* seg,
                                        int64_t stride) {
     for (int c = 0; c < 6; ++c) s.y[c] = ck[c];
@@ -514,9 +692,10 @@ __host__ __device__ inline void replay(const Frame& F, Ray& s, const double* ck,
         double lg;
         const double fac = control_fac(s, A.err, &lg);
         if (A.err <= 1.0) {
-            double* o = seg + static_cast<int64_t>(s.nsteps - n_from) * 5 * stride;
+            double* o = seg + static_cast<int64_t>(s.nsteps - n_from) * kSeg * stride;
             o[0] = s.y[1]; o[stride] = s.y[2]; o[2 * stride] = s.y[4]; o[3 * stride] = s.y[5];
             o[4 * stride] = s.h;
+            o[5 * stride] = near_jet(F, s.y, A.yn) ? 1.0 : 0.0;
             ++s.nsteps;
             s.tries = 0;
             advance(s, A, fac, lg);
@@ -533,12 +712,31 @@ __host__ __device__ inline bool same_ck(const Ray& s, const double* ck) {
     return s.h == ck[6] && s.pw == ck[7];
 }
 
-// Adjoint of w * I for one disc-hit ray that ended after n_acc accepted
-// steps: adds d(w I)/d(spin, theta_cam, eddington, r_cam) to grad.  Returns
-// the number of replayed segments whose end state differed from the next
+// Reverse of one stored step: the adjoint xb of its end state becomes the
+// adjoint of its start; a step near the jet adds its sample's seeds.
+__host__ __device__ inline void bwd_step(const Frame& F, const RayK& K, const double x[4], double h, bool jet,
+                                         double wk, double xb[4], double cb[3]) {
+    double ynb[4] = {xb[0], xb[1], xb[2], xb[3]};
+    double kb[7][4] = {};
+    xb[0] = xb[1] = xb[2] = xb[3] = 0.0;
+    if (jet && wk != 0.0) {
+        double cf[4][5], cbar[5][4] = {};
+        step_dense(K, x, h, cf);
+        jet_seeds(F, cf, h, 1.0, wk, K.pt, K.pph, cbar, cb);
+        dense_coef_vjp(h, cbar, xb, ynb, kb);
+        step_vjp(K, x, h, ynb, kb, true, xb, cb);
+    } else {
+        step_vjp(K, x, h, ynb, kb, false, xb, cb);
+    }
+}
+
+// Adjoint of w * I for one ray that ended after n_acc accepted steps with
+// class cls, I = sigma (g T)^4 / pi of a disc hit + kappa J of the jet:
+// adds d(w I)/d(spin, theta_cam, eddington, r_cam) to grad.  Returns the
+// number of replayed segments whose end state differed from the next
 // checkpoint (0 when the replay reproduces the forward exactly).
-__host__ __device__ inline int bwd_ray(const Frame& F, const RaygenArgs& G, int64_t ray, int n_acc, double w,
-                                       const double* ck, double* seg, int64_t stride, double grad[4]) {
+__host__ __device__ inline int bwd_ray(const Frame& F, const RaygenArgs& G, int64_t ray, int n_acc, int cls,
+                                       double w, const double* ck, double* seg, int64_t stride, double grad[4]) {
     int mismatch = 0;
     int px, py, k;
     ray_pixel(G, ray, px, py, k);
@@ -550,54 +748,68 @@ __host__ __device__ inline int bwd_ray(const Frame& F, const RaygenArgs& G, int6
         camera_photon(G, d, s.pt, pr, pth, s.pph);
     }
     const RayK K = ray_k(F.a, F.a2, s.pt, s.pph);
-    const int n_term = n_acc - 1;           // accepted steps before the terminal one
-    const int j_last = n_term / F.K;
-    // the terminal segment up to the terminal step, then the terminal step itself
-    replay(F, s, ck + j_last * kCk, j_last * F.K, n_term, seg, stride);
-    double xt[4] = {s.y[1], s.y[2], s.y[4], s.y[5]};
-    double se = 0.5, r_hit = 0.0;
-    for (;;) {
-        clamp_h(F, s);
-        Att A;
-        attempt(K, s, F.rtol, F.atol, A);
-        double lg;
-        const double fac = control_fac(s, A.err, &lg);
-        if (A.err <= 1.0) {
-            double cr[5], ct[5];
-            dense_coeffs(s, A, 1, cr);
-            dense_coeffs(s, A, 2, ct);
-            se = root_theta(ct[0], ct[1], ct[2], ct[3], ct[4], s.c_prev, s.y[2], A.yn[2]);
-            r_hit = dense_comp(cr[0], cr[1], cr[2], cr[3], cr[4], se);
-            if (!(s.c_prev * A.g7.cs < 0.0) || !(r_hit >= F.r_in && r_hit <= F.r_out)) ++mismatch;
-            break;
-        }
-        s.h *= dmin(fac, 1.0);
-        if (++s.tries > 40) { ++mismatch; return mismatch; }
-    }
-    const double h_term = s.h;
-    // intensity and its explicit partials at the hit
-    const double I = disc_intensity(F, r_hit, s.pt, s.pph);
-    if (!(I > 0.0)) return mismatch;
-    double dI[4];
-    disc_intensity_grad(F, r_hit, s.pt, s.pph, dI);
+    const double wk = w * F.kappa;       // weight of the jet integral in w I
+    const bool disc = cls == BHX_DISC;
     double xb[4] = {0.0, 0.0, 0.0, 0.0};
-    double cb[3] = {w * dI[1], w * dI[2], w * dI[3]};
-    terminal_vjp(K, xt, h_term, se, w * dI[0], xb, cb);
+    double cb[3] = {0.0, 0.0, 0.0};
+    double I_disc = 0.0, J = 0.0;
+    // the last segment: replayed up to the terminal step (a disc hit) or to
+    // the end (every other class: all steps are full steps)
+    const int n_reg = disc ? n_acc - 1 : n_acc;
+    const int j_last = (n_reg == 0) ? 0 : (n_reg - 1) / F.K;
+    const int last_from = j_last * F.K;
+    replay(F, s, ck + j_last * kCk, last_from, n_reg, seg, stride);
+    if (disc) {
+        const double xt[4] = {s.y[1], s.y[2], s.y[4], s.y[5]};
+        double se = 0.5, r_hit = 0.0;
+        bool jet_term = false;
+        for (;;) {
+            clamp_h(F, s);
+            Att A;
+            attempt(K, s, F.rtol, F.atol, A);
+            double lg;
+            const double fac = control_fac(s, A.err, &lg);
+            if (A.err <= 1.0) {
+                double cr[5], ct[5];
+                dense_coeffs(s, A, 1, cr);
+                dense_coeffs(s, A, 2, ct);
+                se = root_theta(ct[0], ct[1], ct[2], ct[3], ct[4], s.c_prev, s.y[2], A.yn[2]);
+                r_hit = dense_comp(cr[0], cr[1], cr[2], cr[3], cr[4], se);
+                if (!(s.c_prev * A.g7.cs < 0.0) || !(r_hit >= F.r_in && r_hit <= F.r_out)) ++mismatch;
+                jet_term = near_jet(F, s.y, A.yn);
+                break;
+            }
+            s.h *= dmin(fac, 1.0);
+            if (++s.tries > 40) { ++mismatch; return mismatch; }
+        }
+        I_disc = disc_intensity(F, r_hit, s.pt, s.pph);
+        double Or = 0.0;
+        if (I_disc > 0.0) {
+            double dI[4];
+            disc_intensity_grad(F, r_hit, s.pt, s.pph, dI);
+            Or = w * dI[0];
+            cb[0] += w * dI[1]; cb[1] += w * dI[2]; cb[2] += w * dI[3];
+        }
+        J += terminal_vjp(F, K, xt, s.h, se, Or, jet_term, wk, xb, cb);
+    }
     // sweep back through the stored segments
     for (int j = j_last; j >= 0; --j) {
         const int n_from = j * F.K;
-        const int n_to = (j == j_last) ? n_term : n_from + F.K;
+        const int n_to = (j == j_last) ? n_reg : n_from + F.K;
         if (j != j_last) {
             replay(F, s, ck + j * kCk, n_from, n_to, seg, stride);
             if (!same_ck(s, ck + (j + 1) * kCk)) ++mismatch;
         }
         for (int i = n_to - n_from - 1; i >= 0; --i) {
-            const double* o = seg + static_cast<int64_t>(i) * 5 * stride;
+            const double* o = seg + static_cast<int64_t>(i) * kSeg * stride;
             const double xi[4] = {o[0], o[stride], o[2 * stride], o[3 * stride]};
-            double ynb[4] = {xb[0], xb[1], xb[2], xb[3]};
-            double kb[7][4] = {};
-            xb[0] = xb[1] = xb[2] = xb[3] = 0.0;
-            step_vjp(K, xi, o[4 * stride], ynb, kb, false, xb, cb);
+            const bool jet = o[5 * stride] != 0.0;
+            if (jet) {
+                double cf[4][5];
+                step_dense(K, xi, o[4 * stride], cf);
+                J += jet_step(F, cf, o[4 * stride], 1.0, s.pt, s.pph);
+            }
+            bwd_step(F, K, xi, o[4 * stride], jet, wk, xb, cb);
         }
     }
     // camera: x0 = (r_cam, theta_cam, p_r, p_theta) and (p_t, p_phi) as
@@ -614,10 +826,11 @@ __host__ __device__ inline int bwd_ray(const Frame& F, const RaygenArgs& G, int6
     double dp[3];
     for (int m = 0; m < 3; ++m)
         dp[m] = xb[2] * pr.d[m] + xb[3] * pth.d[m] + cb[1] * pt.d[m] + cb[2] * pph.d[m];
-    grad[0] += cb[0] + dp[0];           // spin
-    grad[1] += xb[1] + dp[2];           // theta_cam
-    grad[2] += w * I / F.eddington;     // eddington (F is linear in the accretion rate)
-    grad[3] += xb[0] + dp[1];           // r_cam
+    const double I = I_disc + F.kappa * J;
+    grad[0] += cb[0] + dp[0] + wk * J * F.dlnj0_da;   // spin (incl. the jet power's)
+    grad[1] += xb[1] + dp[2];                         // theta_cam
+    grad[2] += w * I / F.eddington;                   // eddington: disc flux and jet power ~ Mdot
+    grad[3] += xb[0] + dp[1];                         // r_cam
     return mismatch;
 }
 

@@ -33,7 +33,8 @@ T circ_energy_t(const T& a, const T& r, int sg) {
 // Frame constants of scene s with checkpoints every K accepted steps.
 // Returns false for scenes this render cannot differentiate.
 inline bool make_frame(const bhx_scene& s, int K, Frame& F, RaygenArgs& G) {
-    if (!s.disc_enabled || s.ns_enabled || s.jet_enabled || s.sun_enabled || s.hs_enabled) return false;
+    if (!s.disc_enabled || s.ns_enabled || s.sun_enabled || s.hs_enabled) return false;
+    if (s.jet_enabled && s.jet_model != 1) return false;
     if (K < 1 || s.max_steps < 1) return false;
     std::memset(&F, 0, sizeof F);
     G = host::raygen_args(s);
@@ -73,6 +74,35 @@ inline bool make_frame(const bhx_scene& s, int K, Frame& F, RaygenArgs& G) {
                 F.d_c[i] = ci.d[0];
             }
         }
+    }
+    // conical jet: emission j_nu0 x profile x g^(2+alpha) along the ray; the
+    // band integral of its power-law spectrum turns the specific intensity at
+    // nu0 into intensity (the normalisation of bhx_setup_conical_jet)
+    if (s.jet_enabled && s.jet_j_nu0 > 0.0) {
+        const EventParams E = host::event_params(s);
+        F.has_jet = 1;
+        F.j0 = s.jet_j_nu0;
+        F.alpha = s.jet_alpha;
+        F.r_bound = s.jet_r_bound;
+        F.cone_angle = E.cone_angle; F.cone_r0 = E.cone_r0; F.cone_r1 = E.cone_r1;
+        F.cone_beta = E.cone_beta; F.cone_gamma = E.cone_gamma; F.cone_gate = E.cone_gate;
+        const double nu_0 = 299792458.0 / 550e-9, nu_lo = 1.0e9, nu_hi = 1.0e17;
+        const double p = 1.0 - s.jet_alpha;
+        const double band = (std::fabs(p) > 1e-9) ? (std::pow(nu_hi, p) - std::pow(nu_lo, p)) / p
+                                                  : std::log(nu_hi / nu_lo);
+        F.kappa = std::pow(nu_0, s.jet_alpha) * band;
+        // j_nu0 ~ Omega_H^2 f(Omega_H^2) Mdot, Mdot ~ eddington / eta(spin)
+        using D1 = Dual<1>;
+        const double ad = std::clamp(s.spin, -0.9999, 0.9999);
+        const D1 a = D1::var(s.spin, 0);
+        const D1 a_d = (ad == s.spin) ? D1::var(ad, 0) : D1(ad);   // eta is flat beyond the clamp
+        const D1 eta = 1.0 - circ_energy_t(a_d, isco_t(a_d, s.disc_sense), s.disc_sense);
+        const D1 rp = 1.0 + sqrt(1.0 - a * a);
+        const D1 om = a / (2.0 * rp);
+        const D1 x = om * om;
+        const D1 f = 1.0 + 1.38 * x - 9.2 * x * x;
+        const D1 P = x * f / eta;
+        F.dlnj0_da = P.d[0] / P.v;
     }
     // camera frame (src/kerr.h:44-77, 270-293) as a function of (a, r_cam, theta_cam)
     {

@@ -21,6 +21,12 @@ __device__ __forceinline__ double smooth01(double t) {
     return t * t * (3.0 - 2.0 * t);
 }
 
+// Quintic step 6t^5 - 15t^4 + 10t^3 on [0, 1]: continuous second derivative.
+__device__ __forceinline__ double smooth_c2(double t) {
+    t = dclamp(t, 0.0, 1.0);
+    return t * t * t * (10.0 + t * (6.0 * t - 15.0));
+}
+
 // Jet::shape (jet.h:191-203): dimensionless comoving emissivity at (z, rho).
 __device__ inline double jet_shape(const EventParams& J, double z, double rho) {
     const double az = fabs(z);
@@ -80,16 +86,18 @@ __device__ inline double jet_boost(const EventParams& J, double a, double a2, do
 
 // Conical jet (model 1, no counterpart in the reference): comoving emissivity
 // profile at (r, cos theta) - (r0 / r)^2 inside two cones of half opening
-// cone_angle about the spin axis, smooth over the outer fifth of the opening
-// and at both ends (the end ramps of the reference's jets, jet.h:199-202).
+// cone_angle about the spin axis, tapered over the outer fifth of the opening
+// and ramped at both ends (the end ramps of the reference's jets,
+// jet.h:199-202) with the quintic step, so that the profile has a continuous
+// second derivative and its path integral a smooth gradient.
 __device__ inline double cone_shape(const EventParams& J, double r, double cs) {
     if (r <= J.cone_r0 || r >= J.cone_r1) return 0.0;
     const double psi = acos(dmin(fabs(cs), 1.0));
-    const double ta = smooth01((J.cone_angle - psi) / (0.2 * J.cone_angle));
+    const double ta = smooth_c2((J.cone_angle - psi) / (0.2 * J.cone_angle));
     if (ta <= 0.0) return 0.0;
     const double q = J.cone_r0 / r;
-    const double ramp = smooth01((r - J.cone_r0) / (0.5 * J.cone_r0)) *
-                        smooth01((J.cone_r1 - r) / (0.15 * J.cone_r1));
+    const double ramp = smooth_c2((r - J.cone_r0) / (0.5 * J.cone_r0)) *
+                        smooth_c2((J.cone_r1 - r) / (0.15 * J.cone_r1));
     return q * q * ta * ramp;
 }
 

@@ -43,6 +43,11 @@ double smooth01(double t) {
     return t * t * (3.0 - 2.0 * t);
 }
 
+double smooth_c2(double t) {   // the conical jet's quintic step
+    t = std::clamp(t, 0.0, 1.0);
+    return t * t * t * (10.0 + t * (6.0 * t - 15.0));
+}
+
 double horizon_omega(double a) {  // kerr.h:280-283
     const double rp = 1.0 + std::sqrt(std::max(0.0, 1.0 - a * a));
     return (rp > 0.0) ? a / (2.0 * rp) : 0.0;
@@ -265,7 +270,7 @@ int bhx_setup_conical_jet(bhx_scene* s, double phi_flux, double eps_rad) {
         const double dpsi = th_j / N;
         for (int i = 0; i < N; ++i) {
             const double psi = (i + 0.5) * dpsi;
-            ang += smooth01((th_j - psi) / (0.2 * th_j)) * std::sin(psi) * dpsi;
+            ang += smooth_c2((th_j - psi) / (0.2 * th_j)) * std::sin(psi) * dpsi;
         }
     }
     double rad = 0.0;
@@ -275,7 +280,7 @@ int bhx_setup_conical_jet(bhx_scene* s, double phi_flux, double eps_rad) {
         for (int i = 0; i < N; ++i) {
             const double r = r0 + (i + 0.5) * dr;
             const double q = r0 / r;
-            rad += q * q * smooth01((r - r0) / (0.5 * r0)) * smooth01((r1 - r) / (0.15 * r1)) * r * r * dr;
+            rad += q * q * smooth_c2((r - r0) / (0.5 * r0)) * smooth_c2((r1 - r) / (0.15 * r1)) * r * r * dr;
         }
     }
     const double V = 2.0 * (2.0 * kPi) * ang * rad;

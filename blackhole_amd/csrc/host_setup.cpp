@@ -271,6 +271,7 @@ EventParams event_params(const bhx_scene& s) {
     E.cone_r1 = s.jet_cone_r1;
     E.cone_gamma = std::max(1.0, s.jet_gamma_max);
     E.cone_beta = std::sqrt(std::max(0.0, 1.0 - 1.0 / (E.cone_gamma * E.cone_gamma)));
+    E.cone_gate = std::cos(std::min(s.jet_cone_angle + kConeGateMargin, 1.5707963267948966));
     return E;
 }
 

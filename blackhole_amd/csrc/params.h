@@ -11,10 +11,19 @@ struct EventParams {
     double z_base, z_top, R_base, k, gamma_max, z_sat, alpha, u_peak, u_wid, spine, spine_wid;
     double zeta, Omega_F, j_nu0, r_bound, gamma_zb;   // gamma_zb = Jet::gamma_at(z_base)
     double sun_c[3], sun_R;
-    // conical jet (model 1): half opening, radial extent, outflow speed and Lorentz factor
-    double cone_angle, cone_r0, cone_r1, cone_beta, cone_gamma;
+    // conical jet (model 1): half opening, radial extent, outflow speed and
+    // Lorentz factor; cone_gate = cos(min(cone_angle + kConeGateMargin, pi/2))
+    double cone_angle, cone_r0, cone_r1, cone_beta, cone_gamma, cone_gate;
     int32_t model, pad;
 };
+
+// Sampling of the conical jet along a step: kConeSub sub-intervals of
+// 4-point Gauss-Legendre each, on steps with an end within cone_angle +
+// kConeGateMargin [rad] of the spin axis.  A step moves the polar angle by
+// less than 0.1 rad (h <= 0.05 (r - r_h) + 0.01), so a step with both ends
+// outside the gate lies outside the cone, where the emission is zero.
+constexpr int kConeSub = 8;
+constexpr double kConeGateMargin = 0.2;
 
 // One scene of a batched launch: the tracer's per-scene constants (the
 // uniform fields of TraceArgs) plus its event parameters.  The batched tracer

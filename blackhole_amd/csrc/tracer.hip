@@ -282,6 +282,10 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
                 to_cart(la2, y[1], y[2], y[3], p0);
                 to_cart(la2, yn[1], yn[2], yn[3], p1);
                 if (SK(has_jet)) near_jet = seg_dist(p0, p1) < EV.r_bound;          // render.cpp:151-152
+                // conical jet: only steps with an end inside the sampling gate
+                // (z = r cos theta) can reach the cone
+                if (near_jet && EV.model == 1)
+                    near_jet = fabs(p0[2]) > EV.cone_gate * y[1] || fabs(p1[2]) > EV.cone_gate * yn[1];
                 if (SK(has_sun) && !ns_hit && !horizon) {                             // render.cpp:203-209
                     const double* C = EV.sun_c;
                     const double e0 = p1[0] - p0[0], e1 = p1[1] - p0[1], e2 = p1[2] - p0[2];
@@ -364,7 +368,29 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
                     }
                 }
                 if (term < 0) s_jet = 1.0;   // nothing stopped the ray: the whole step counts
-                if (near_jet && s_jet > 0.0) {
+                if (near_jet && s_jet > 0.0 && EV.model == 1) {
+                    // ---- conical jet: kConeSub x 4-point Gauss-Legendre
+                    const EventParams& J = EV;
+                    double sum = 0.0;
+#pragma unroll 1
+                    for (int ks = 0; ks < kConeSub; ++ks) {
+#pragma unroll
+                        for (int qd = 0; qd < 4; ++qd) {
+                            const double sv = (ks + kGLx[qd]) * (1.0 / kConeSub) * s_jet;
+                            const double rs = dense_comp(c1[1], c2[1], c3[1], c4[1], c5[1], sv);
+                            const double ths = dense_comp(c1[2], c2[2], c3[2], c4[2], c5[2], sv);
+                            double sn, cs;
+                            sincos_near(ths, &sn, &cs);
+                            const double sh = cone_shape(J, rs, cs);
+                            if (sh <= 0.0) continue;
+                            const double prs = dense_comp(c1[4], c2[4], c3[4], c4[4], c5[4], sv);
+                            const double b = cone_boost(J, la, la2, rs, ths, pt, prs, pph);
+                            if (b <= 0.0) continue;
+                            sum += kGLw[qd] * sh * b;
+                        }
+                    }
+                    jet_acc += J.j_nu0 * sum * h * s_jet * (1.0 / kConeSub);
+                } else if (near_jet && s_jet > 0.0) {
                     // ---- jet emission: 4-point Gauss-Legendre (render.cpp:107-122)
                     const EventParams& J = EV;
                     double sum = 0.0;
@@ -375,14 +401,11 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
                         const double ths = dense_comp(c1[2], c2[2], c3[2], c4[2], c5[2], sv);
                         double sn, cs;
                         sincos_near(ths, &sn, &cs);
-                        const double sh = (J.model == 1) ? cone_shape(J, rs, cs)
-                                                         : jet_shape(J, rs * cs, sqrt(rs * rs + la2) * sn);
+                        const double sh = jet_shape(J, rs * cs, sqrt(rs * rs + la2) * sn);
                         if (sh <= 0.0) continue;
                         const double prs = dense_comp(c1[4], c2[4], c3[4], c4[4], c5[4], sv);
-                        const double b = (J.model == 1)
-                            ? cone_boost(J, la, la2, rs, ths, pt, prs, pph)
-                            : jet_boost(J, la, la2, rs, ths, pt, prs,
-                                        dense_comp(c1[5], c2[5], c3[5], c4[5], c5[5], sv), pph);
+                        const double b = jet_boost(J, la, la2, rs, ths, pt, prs,
+                                                   dense_comp(c1[5], c2[5], c3[5], c4[5], c5[5], sv), pph);
                         if (b <= 0.0) continue;
                         sum += kGLw[qd] * sh * b;
                     }

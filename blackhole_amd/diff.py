@@ -26,12 +26,25 @@ from .abi import Scene
 PARAMS = ("spin", "cam_theta", "eddington", "cam_r")
 
 
+def _unit_cone_jet(s):
+    """s re-normalised in place for phi_flux = eps_rad = 1; its j_nu0."""
+    native.check(None, native.lib().bhx_setup_conical_jet(C.byref(s), 1.0, 1.0), "bhx_setup_conical_jet")
+    return s.jet_j_nu0
+
+
 def with_params(scene, p):
     """Copy of scene with (spin, cam_theta, eddington, cam_r) = p; r_escape
-    and the rest of the camera stay as they are."""
+    and the rest of the camera stay as they are.  The conical jet's emission,
+    j_nu0 ~ phi_flux^2 eps_rad x (a function of spin and accretion rate), is
+    carried to the new spin and rate at the scene's phi_flux^2 eps_rad."""
     s = Scene()
     C.pointer(s)[0] = scene
     s.spin, s.cam_theta, s.eddington, s.cam_r = (float(v) for v in p)
+    if s.jet_enabled and s.jet_model == 1 and scene.jet_j_nu0 > 0.0:
+        old = Scene()
+        C.pointer(old)[0] = scene
+        scale = scene.jet_j_nu0 / _unit_cone_jet(old)
+        s.jet_j_nu0 = _unit_cone_jet(s) * scale
     return s
 
 
@@ -92,7 +105,7 @@ class DiscRender:
         f, b = C.c_double(), C.c_double()
         native.check(self.tr.ctx, self.lib.bhx_adjoint_info(self.adj, C.byref(ck), C.byref(disc), C.byref(mism),
                                                             C.byref(f), C.byref(b)), "bhx_adjoint_info")
-        return {"ck_bytes": ck.value, "disc_rays": disc.value, "mismatches": mism.value,
+        return {"ck_bytes": ck.value, "lit_rays": disc.value, "mismatches": mism.value,
                 "fwd_ms": f.value, "bwd_ms": b.value}
 
     # ---- autograd ------------------------------------------------------------

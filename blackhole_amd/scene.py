@@ -213,6 +213,17 @@ def quasar_frame(incl_deg, width=1024, height=1024, stars=True, half_extent=1050
     return s
 
 
+def quasar_diff_frame(incl_deg, width=64, height=64, half_extent=120.0, rtol=1e-10, atol=1e-12):
+    """The quasar frame as a differentiation scene: no stars (the adjoint's
+    intensity is the disc's and the jet's), tolerances 1e-10 / 1e-12 and
+    r_escape fixed at 3 * 2500 + 100 so that varying r_cam does not move the
+    escape sphere."""
+    s = quasar_frame(incl_deg, width, height, stars=False, half_extent=half_extent)
+    s.rtol, s.atol = rtol, atol
+    s.r_escape = 3.0 * 2500.0 + 100.0
+    return s
+
+
 def quasar_sweep(n_frames=120, width=1024, height=1024, stars=True):
     """Inclination swept linearly 0 -> 90 deg over n_frames (clamped at 0.5
     deg like the CLI, src/main.cpp:810)."""

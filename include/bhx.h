@@ -265,17 +265,21 @@ int bhx_batch_autotune(bhx_batch *batch, int32_t reps, bhx_launch *best, double 
 
 /* ---- differentiable render ----------------------------------------------
  *
- * The bolometric disc image I = sigma (g T)^4 / pi per sample (0 off the
- * disc; the disc model of src/disc.h:94-121, src/render.cpp:280-300) and its
- * discrete adjoint: given dL/dI per sample, the gradient of L with respect to
- * (spin, cam_theta [rad], eddington, cam_r).  The forward records a
- * checkpoint of the integrator state every K accepted steps (HBM: samples x
- * (max_steps / K + 1) x 64 bytes); the backward replays each K-step segment
- * from its checkpoint with the same arithmetic and sweeps it in reverse with
- * the step sizes frozen, and differentiates the disc crossing time by the
- * implicit-function theorem.  No counterpart in the reference, which has no
- * derivatives; the forward image is the reference's (same tracer).  Scenes
- * with a neutron star, jets, the Sun or a hot spot are rejected.
+ * The intensity image I per sample [W m^-2 sr^-1] - the bolometric disc
+ * intensity sigma (g T)^4 / pi of a disc hit (the disc model of
+ * src/disc.h:94-121, src/render.cpp:280-300) plus, with the conical jet
+ * (jet_model 1), its band-integrated intensity kappa J, J the specific
+ * intensity at nu0 integrated along the ray - and its discrete adjoint: given
+ * dL/dI per sample, the gradient of L with respect to (spin, cam_theta [rad],
+ * eddington, cam_r).  The forward records a checkpoint of the integrator
+ * state every K accepted steps (HBM: samples x (max_steps / K + 1) x 64
+ * bytes); the backward replays each K-step segment from its checkpoint with
+ * the same arithmetic and sweeps it in reverse with the step sizes frozen,
+ * and differentiates the disc crossing time by the implicit-function
+ * theorem.  No counterpart in the reference, which has no derivatives; the
+ * forward image is the reference's (same tracer).  Scenes with a neutron
+ * star, the reference's jets (jet_model 0), the Sun or a hot spot are
+ * rejected.
  */
 typedef struct bhx_adjoint bhx_adjoint;
 
@@ -284,8 +288,8 @@ int bhx_adjoint_destroy(bhx_adjoint *adj);
 /* New parameters for the same frame size and max_steps (synchronises). */
 int bhx_adjoint_set_scene(bhx_adjoint *adj, const bhx_scene *scene);
 /* Recording forward: I (device, one double per sample) and the checkpoints,
- * then the backward's ray order (disc hits by step count, a device radix
- * sort).  Stream-ordered, no host synchronisation. */
+ * then the backward's ray order (lit rays, I > 0, by step count, a device
+ * radix sort).  Stream-ordered, no host synchronisation. */
 int bhx_adjoint_forward(bhx_adjoint *adj, double *I, void *stream);
 /* w (device, one double per sample) = dL/dI; grad (host, 4 doubles) =
  * dL/d(spin, cam_theta, eddington, cam_r).  Synchronous. */
@@ -294,9 +298,9 @@ int bhx_adjoint_backward(bhx_adjoint *adj, const double *w, double *grad, void *
 int bhx_adjoint_rays(const bhx_adjoint *adj, bhx_rays *rays);
 /* The same copied to HOST buffers (any may be NULL; synchronises `stream`). */
 int bhx_adjoint_copy_rays(bhx_adjoint *adj, uint8_t *cls, double *y, int32_t *steps, void *stream);
-/* Checkpoint slab size, disc-hit rays of the last forward, replayed segments
+/* Checkpoint slab size, lit rays (I > 0) of the last forward, replayed segments
  * that did not reproduce their checkpoint (0 expected), kernel times. */
-int bhx_adjoint_info(bhx_adjoint *adj, int64_t *ck_bytes, int64_t *disc_rays, int64_t *mismatches,
+int bhx_adjoint_info(bhx_adjoint *adj, int64_t *ck_bytes, int64_t *lit_rays, int64_t *mismatches,
                      double *fwd_ms, double *bwd_ms);
 
 /* ---- scene preparation (host only, no context, no GPU) ------------------ */

@@ -14,6 +14,7 @@ Parity status per function: every function of src/oracle.cpp is pinned, see
 the header of src/oracle.cpp and tests/test_oracle_ref.py / test_oracle_golden.py.
 """
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -200,9 +201,21 @@ def ref_trace_list(scene, ray_ids):
     return out
 
 
+def jet_band_factor(alpha, nu_0=299792458.0 / 550e-9, nu_lo=1.0e9, nu_hi=1.0e17):
+    """kappa = nu_0^alpha * integral of nu^-alpha over [nu_lo, nu_hi]: the
+    intensity of a power-law spectrum I_nu = J (nu / nu_0)^-alpha over the
+    band, per unit specific intensity J at nu_0 (the band of the conical
+    jet's normalisation)."""
+    p = 1.0 - alpha
+    band = (nu_hi ** p - nu_lo ** p) / p if abs(p) > 1e-9 else math.log(nu_hi / nu_lo)
+    return nu_0 ** alpha * band
+
+
 def disc_intensity(scene, threads=0):
-    """Every ray of the scene traced by the oracle, and the bolometric
-    intensity sigma (g T)^4 / pi of its disc hits (0 elsewhere)."""
+    """Every ray of the scene traced by the oracle, and its intensity: the
+    bolometric sigma (g T)^4 / pi of a disc hit (0 elsewhere) plus, with the
+    conical jet (jet_model 1), kappa times the jet's specific intensity at
+    nu_0 integrated along the ray."""
     ids = np.arange(scene.n_rays)
     c = trace_list(scene, ids, threads=threads)
     cls = np.ascontiguousarray(c["cls"], dtype=np.uint8)
@@ -210,6 +223,8 @@ def disc_intensity(scene, threads=0):
     g = np.ascontiguousarray(c["g"])
     out = np.zeros(scene.n_rays)
     lib().oracle_disc_intensity(C.byref(scene), C.c_int64(scene.n_rays), _ptr(cls), _ptr(r), _ptr(g), _ptr(out))
+    if scene.jet_enabled and scene.jet_model == 1:
+        out = out + jet_band_factor(scene.jet_alpha) * c["jet"]
     return out, c
 
 

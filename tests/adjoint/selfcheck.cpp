@@ -6,6 +6,8 @@
 //   term   the terminal step: r at the equatorial crossing of the dense output
 //   ray    d I / d(spin, theta_cam, eddington, r_cam) of whole disc-hit rays
 //          against finite differences of the full adaptive forward
+//   jet    the same for a scene with the conical jet: rays lit by the jet
+//          (any class) and disc hits seen through it
 // Prints one line per check: name, count, worst relative error.
 #include "../../blackhole_amd/csrc/adjoint_frame.h"
 
@@ -145,7 +147,8 @@ static void check_terminal(std::mt19937_64& rng) {
         double r0, s0;
         if (!cross_r(K, x, h, &r0, &s0)) continue;
         double xb[4] = {}, cb[3] = {};
-        terminal_vjp(K, x, h, s0, 1.0, xb, cb);
+        const Frame F0{};   // no jet: the seed is r(s*) alone
+        terminal_vjp(F0, K, x, h, s0, 1.0, false, 0.0, xb, cb);
         double fd[7];
         bool ok = true;
         auto R = [&](const double* xx, double aa, double ptt, double pphh, double* out) {
@@ -176,7 +179,18 @@ static void check_terminal(std::mt19937_64& rng) {
     std::printf("term %d %.3e\n", n, worst);
 }
 
-// Full adaptive forward of one ray: its intensity (0 unless a disc hit).
+// The differentiable parameters (spin, theta_cam, eddington, r_cam) of a
+// scene; a scene with the conical jet is re-normalised for a new spin and
+// accretion rate, as the jet's power depends on both.
+static bhx_scene with_param(const bhx_scene& sc, int p, double v) {
+    bhx_scene s = sc;
+    double* f = p == 0 ? &s.spin : p == 1 ? &s.cam_theta : p == 2 ? &s.eddington : &s.cam_r;
+    *f = v;
+    if (s.jet_enabled && s.jet_model == 1) bhx_setup_conical_jet(&s, 50.0, 0.02);
+    return s;
+}
+
+// Full adaptive forward of one ray: its intensity (disc plus jet).
 static double ray_I(const bhx_scene& sc, int K, int64_t ray, int* cls, int* n_acc) {
     Frame F;
     RaygenArgs G;
@@ -189,58 +203,85 @@ static double ray_I(const bhx_scene& sc, int K, int64_t ray, int* cls, int* n_ac
     while (term < 0) term = fwd_step(F, s, ck.data(), yo, &se);
     *cls = term;
     *n_acc = s.nsteps;
-    return term == BHX_DISC ? disc_intensity(F, yo[1], s.pt, s.pph) : 0.0;
+    return (term == BHX_DISC ? disc_intensity(F, yo[1], s.pt, s.pph) : 0.0) + F.kappa * s.jet;
 }
 
-static void check_ray(double rtol, double atol, int K) {
-    bhx_scene sc{};
-    // a 512^2-like differentiation scene, sampled: a = 0.7, i = 45 deg, r = 500 M
-    sc.M_kg = 10.0 * 1.98847e30; sc.spin = 0.7;
-    sc.cam_r = 500.0; sc.cam_theta = 45.0 * 3.14159265358979323846 / 180.0; sc.cam_fov = 2.0 * std::atan(20.0 / 500.0);
-    sc.width = 64; sc.height = 64; sc.sqrt_spp = 1; sc.max_steps = 20000;
-    sc.disc_enabled = 1; sc.disc_r_out = 20.0; sc.eddington = 0.1; sc.disc_sense = 1;
-    sc.rtol = rtol; sc.atol = atol; sc.r_escape = 3.0 * 500.0 + 100.0; sc.seed = 20240921ull;
+// Whole rays of scene sc (every `stride`-th ray with I > 0, up to n_max):
+// the adjoint's d I / d(spin, theta_cam, eddington, r_cam) against central
+// differences (step 1e-6 max(1, |p|)) of the full adaptive forward, over rays
+// whose class does not change.
+static void check_ray(const char* name, const bhx_scene& sc, int K, int64_t stride, int n_max) {
     Frame F;
     RaygenArgs G;
-    make_frame(sc, K, F, G);
-    std::vector<double> ck(static_cast<size_t>(F.cap) * kCk), seg(static_cast<size_t>(K) * 5);
+    if (!make_frame(sc, K, F, G)) {
+        std::printf("%s 0 1 mismatch 0\n", name);
+        return;
+    }
+    std::vector<double> ck(static_cast<size_t>(F.cap) * kCk), seg(static_cast<size_t>(K) * kSeg);
     double worst = 0.0;
-    int n = 0, mism = 0;
-    for (int64_t ray = 0; ray < (int64_t)sc.width * sc.height && n < 40; ray += 37) {
-        int cls, n_acc;
+    int n = 0, mism = 0, n_jet = 0;
+    for (int64_t ray = 0; ray < (int64_t)sc.width * sc.height && n < n_max; ray += stride) {
         Ray s;
         fwd_init(F, G, ray, s, ck.data());
         double yo[6], se = 0.0;
         int term = -1;
         while (term < 0) term = fwd_step(F, s, ck.data(), yo, &se);
-        if (term != BHX_DISC) continue;
-        const double I0 = disc_intensity(F, yo[1], s.pt, s.pph);
+        const double I0 = (term == BHX_DISC ? disc_intensity(F, yo[1], s.pt, s.pph) : 0.0) + F.kappa * s.jet;
         if (!(I0 > 0.0)) continue;
         double grad[4] = {0, 0, 0, 0};
-        mism += bwd_ray(F, G, ray, s.nsteps, 1.0, ck.data(), seg.data(), 1, grad);
-        // central differences of the adaptive forward, relative step 1e-6
+        mism += bwd_ray(F, G, ray, s.nsteps, term, 1.0, ck.data(), seg.data(), 1, grad);
         double fd[4];
         bool ok = true;
         for (int p = 0; p < 4; ++p) {
-            bhx_scene a = sc, b = sc;
-            double* fa = p == 0 ? &a.spin : p == 1 ? &a.cam_theta : p == 2 ? &a.eddington : &a.cam_r;
-            double* fb = p == 0 ? &b.spin : p == 1 ? &b.cam_theta : p == 2 ? &b.eddington : &b.cam_r;
-            const double d = 1e-6 * std::fabs(*fa);
-            *fa += d; *fb -= d;
+            const double v = p == 0 ? sc.spin : p == 1 ? sc.cam_theta : p == 2 ? sc.eddington : sc.cam_r;
+            const double d = 1e-6 * std::max(1.0, std::fabs(v));   // theta_cam can be a few degrees
             int ca, cb_, na, nb;
-            const double Ia = ray_I(a, K, ray, &ca, &na), Ib = ray_I(b, K, ray, &cb_, &nb);
-            if (ca != BHX_DISC || cb_ != BHX_DISC) ok = false;
+            const double Ia = ray_I(with_param(sc, p, v + d), K, ray, &ca, &na);
+            const double Ib = ray_I(with_param(sc, p, v - d), K, ray, &cb_, &nb);
+            if (ca != term || cb_ != term) ok = false;
             fd[p] = (Ia - Ib) / (2 * d);
         }
         if (!ok) continue;
         double e = 0.0;
         for (int p = 0; p < 4; ++p) e = std::max(e, rel(grad[p], fd[p], 1e-30));
-        if (e > 1e-3) std::printf("  ray %ld steps %d grad %.6e %.6e %.6e %.6e fd %.6e %.6e %.6e %.6e\n", (long)ray,
-                                  s.nsteps, grad[0], grad[1], grad[2], grad[3], fd[0], fd[1], fd[2], fd[3]);
+        if (e > 1e-3)
+            std::printf("  ray %ld cls %d steps %d jet/I %.3e grad %.6e %.6e %.6e %.6e fd %.6e %.6e %.6e %.6e\n",
+                        (long)ray, term, s.nsteps, F.kappa * s.jet / I0, grad[0], grad[1], grad[2], grad[3], fd[0],
+                        fd[1], fd[2], fd[3]);
         worst = std::max(worst, e);
+        if (s.jet > 0.0) ++n_jet;
         ++n;
     }
-    std::printf("ray_rtol%.0e_K%d %d %.3e mismatch %d\n", rtol, K, n, worst, mism);
+    std::printf("%s %d %.3e mismatch %d jet %d\n", name, n, worst, mism, n_jet);
+}
+
+// The differentiation scene, sampled: a = 0.7, i = 45 deg, r = 500 M, +-20 M.
+static bhx_scene disc_scene(double rtol, double atol) {
+    bhx_scene sc{};
+    sc.M_kg = 10.0 * 1.98847e30; sc.spin = 0.7;
+    sc.cam_r = 500.0; sc.cam_theta = 45.0 * 3.14159265358979323846 / 180.0; sc.cam_fov = 2.0 * std::atan(20.0 / 500.0);
+    sc.width = 64; sc.height = 64; sc.sqrt_spp = 1; sc.max_steps = 20000;
+    sc.disc_enabled = 1; sc.disc_r_out = 20.0; sc.eddington = 0.1; sc.disc_sense = 1;
+    sc.rtol = rtol; sc.atol = atol; sc.r_escape = 3.0 * 500.0 + 100.0; sc.seed = 20240921ull;
+    return sc;
+}
+
+// A quasar frame zoomed on the jets' base: a = 0.998, 1000 M disc at 0.5
+// Eddington, conical jet (5 deg, Gamma 10, 6-500 M), camera at 2500 M and
+// inclination incl_deg over +-half M.
+static bhx_scene jet_scene(double incl_deg, double half, double rtol, double atol) {
+    bhx_scene sc{};
+    sc.M_kg = 10.0 * 1.98847e30; sc.spin = 0.998;
+    sc.cam_r = 2500.0; sc.cam_theta = incl_deg * 3.14159265358979323846 / 180.0;
+    sc.cam_fov = 2.0 * std::atan(half / 2500.0);
+    sc.width = 48; sc.height = 48; sc.sqrt_spp = 1; sc.max_steps = 20000;
+    sc.disc_enabled = 1; sc.disc_r_out = 1000.0; sc.eddington = 0.5; sc.disc_sense = 1;
+    sc.jet_enabled = 1; sc.jet_model = 1;
+    sc.jet_cone_angle = 5.0 * 3.14159265358979323846 / 180.0; sc.jet_cone_r0 = 6.0; sc.jet_cone_r1 = 500.0;
+    sc.jet_gamma_max = 10.0; sc.jet_alpha = 0.7;
+    bhx_setup_conical_jet(&sc, 50.0, 0.02);
+    sc.rtol = rtol; sc.atol = atol; sc.r_escape = 3.0 * 2500.0 + 100.0; sc.seed = 20240921ull;
+    return sc;
 }
 
 int main() {
@@ -248,7 +289,9 @@ int main() {
     check_rhs(rng);
     check_step(rng);
     check_terminal(rng);
-    check_ray(1e-9, 1e-11, 16);
-    check_ray(1e-12, 1e-14, 7);
+    check_ray("ray_rtol1e-09_K16", disc_scene(1e-9, 1e-11), 16, 37, 40);
+    check_ray("ray_rtol1e-12_K7", disc_scene(1e-12, 1e-14), 7, 37, 40);
+    check_ray("jet_i10_K16", jet_scene(10.0, 120.0, 1e-10, 1e-12), 16, 7, 80);
+    check_ray("jet_i3_K7", jet_scene(3.0, 120.0, 1e-12, 1e-14), 7, 13, 80);
     return 0;
 }

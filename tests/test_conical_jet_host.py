@@ -1,0 +1,103 @@
+"""Host normalisation of the conical jet (bhx_setup_conical_jet, the build's
+own jet model: no reference counterpart) and the band factor that turns its
+specific intensity into intensity.  CPU only.
+
+The normalisation spreads the radiated fraction eps_rad of the
+Blandford-Znajek power over both cones: 4 pi j_nu0 nu_0^alpha band V = L_rad
+(V the profile-weighted volume).  Pinned by
+* the spin and accretion-rate dependence of the reference's own jet
+  normalisation (bhx_setup_jet, itself pinned against the reference's values
+  in test_host_physics): both take L_rad from the same Blandford-Znajek power,
+  and neither volume depends on spin or rate;
+* the volume integral done independently by scipy's adaptive quadrature of
+  the same profile (the library sums on a midpoint grid);
+* the band factor against scipy's quadrature of nu^-alpha.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+import oracle
+from blackhole_amd import scene as S
+
+G, C_LIGHT, SIGMA_T, M_P = 6.67430e-11, 2.99792458e8, 6.6524587321e-29, 1.67262192369e-27
+
+
+def _cone(spin=0.998, edd=0.5, phi=50.0, eps=0.02, M=10.0 * S.M_SUN):
+    sc = S.default_scene()
+    sc.M_kg, sc.spin, sc.eddington, sc.disc_enabled, sc.disc_r_out = M, spin, edd, 1, 1000.0
+    S.setup_conical_jet(sc, phi_flux=phi, eps_rad=eps)
+    return sc
+
+
+def _ref_jet(spin, edd=0.5, M=10.0 * S.M_SUN):
+    sc = S.default_scene()
+    sc.M_kg, sc.spin, sc.eddington, sc.disc_enabled, sc.disc_r_out, sc.jet_enabled = M, spin, edd, 1, 1000.0, 1
+    S.setup_jet(sc)
+    return sc.jet_j_nu0
+
+
+def _smooth_c2(t):
+    t = min(max(t, 0.0), 1.0)
+    return t * t * t * (10.0 + t * (6.0 * t - 15.0))
+
+
+def test_band_factor_against_quadrature():
+    nu0 = C_LIGHT / 550e-9
+    for alpha in (0.0, 0.5, 0.7, 1.0, 1.5):
+        # integral of nu^-alpha d nu = integral of e^((1-alpha) u) du, u = ln nu
+        band, _ = integrate.quad(lambda u: math.exp((1.0 - alpha) * u), math.log(1e9), math.log(1e17),
+                                 epsabs=0.0, epsrel=1e-13, limit=200)
+        assert oracle.jet_band_factor(alpha) == pytest.approx(nu0 ** alpha * band, rel=1e-11), alpha
+
+
+def test_spin_and_rate_dependence_follow_the_reference_jet():
+    base = _cone().jet_j_nu0
+    ref_base = _ref_jet(0.998)
+    for spin in (0.3, 0.7, 0.9):
+        assert _cone(spin=spin).jet_j_nu0 / base == pytest.approx(_ref_jet(spin) / ref_base, rel=1e-12)
+    assert _cone(edd=0.05).jet_j_nu0 / base == pytest.approx(0.1, rel=1e-13)
+    assert _cone(phi=25.0).jet_j_nu0 / base == pytest.approx(0.25, rel=1e-13)
+    assert _cone(eps=0.2).jet_j_nu0 / base == pytest.approx(10.0, rel=1e-13)
+
+
+def test_radiated_power_identity():
+    """4 pi j_nu0 nu0^alpha band V r_g^2 = eps_rad P_BZ, V by scipy."""
+    sc = _cone()
+    th, r0, r1 = sc.jet_cone_angle, sc.jet_cone_r0, sc.jet_cone_r1
+    ang, _ = integrate.quad(lambda p: _smooth_c2((th - p) / (0.2 * th)) * math.sin(p), 0.0, th,
+                            points=[0.8 * th], epsabs=0.0, epsrel=1e-12, limit=200)
+    rad, _ = integrate.quad(lambda r: (r0 / r) ** 2 * _smooth_c2((r - r0) / (0.5 * r0))
+                            * _smooth_c2((r1 - r) / (0.15 * r1)) * r * r, r0, r1,
+                            points=[1.5 * r0, 0.85 * r1], epsabs=0.0, epsrel=1e-12, limit=400)
+    V = 2.0 * 2.0 * math.pi * ang * rad                        # both cones, units of (GM/c^2)^3
+    r_g = G * sc.M_kg / C_LIGHT ** 2
+    # the Blandford-Znajek power of the reference's jets (jet.h:82, src/main.cpp:399-423)
+    a = sc.spin
+    a_d = min(max(a, -0.9999), 0.9999)
+    r_isco = oracle.lib().oracle_isco(a_d, 1)
+    sr = math.sqrt(r_isco)
+    E = (r_isco ** 2 - 2 * r_isco + a_d * sr) / (r_isco * math.sqrt(r_isco ** 2 - 3 * r_isco + 2 * a_d * sr))
+    L_edd = 4 * math.pi * G * sc.M_kg * M_P * C_LIGHT / SIGMA_T
+    mdot_c2 = sc.eddington * L_edd / (1.0 - E)
+    om = a / (2.0 * (1.0 + math.sqrt(1.0 - a * a)))
+    x = om * om
+    P_bz = 0.053 / (4 * math.pi) * 50.0 ** 2 * x * (1 + 1.38 * x - 9.2 * x * x) * mdot_c2
+    L = 4 * math.pi * sc.jet_j_nu0 * oracle.jet_band_factor(sc.jet_alpha) * V * r_g ** 2
+    # midpoint sums of 400 x 8000 cells against adaptive quadrature
+    assert L == pytest.approx(0.02 * P_bz, rel=2e-6)
+    assert sc.jet_r_bound == pytest.approx(1.05 * r1, rel=1e-15)
+    assert sc.jet_model == 1 and sc.jet_enabled == 1
+
+
+def test_switch_offs():
+    sc = S.default_scene()
+    sc.disc_enabled = 0
+    S.setup_conical_jet(sc)
+    assert sc.jet_enabled == 0
+    sc = S.default_scene()
+    sc.disc_enabled, sc.spin = 1, 0.0           # no rotational energy to tap
+    S.setup_conical_jet(sc)
+    assert sc.jet_enabled == 0
