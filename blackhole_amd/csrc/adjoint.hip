@@ -11,7 +11,7 @@
 //                   cap = max_steps / K + 1 records of 8 doubles per ray).
 //   k_adj_backward  one lit ray (I > 0) per lane, sorted by step count so the
 //                   lanes of a wave replay and sweep segments of equal length
-//                   together; each lane's K-step segment buffer is a strided
+//                   together; each lane's (K + 1)-step segment buffer is a strided
 //                   HBM scratch.  Per-block partial gradients, summed on the
 //                   host in a fixed order.
 #include "adjoint_frame.h"
@@ -44,7 +44,7 @@ struct AdjArgs {
     // backward
     const double* w;          // dL/dI per sample
     const int32_t* order;     // rays by key, descending: the lit rays first
-    double* seg;              // K * kSeg doubles per lane, strided by seg_stride
+    double* seg;              // (K + 1) * kSeg doubles per lane, strided by seg_stride
     int64_t seg_stride;
     double* partial;          // 4 per block
     unsigned int* mismatch;
@@ -210,7 +210,7 @@ int bhx_adjoint_create(bhx_context* ctx, const bhx_scene* s, int32_t K, bhx_adjo
         A->steps.ensure(sizeof(int32_t) * n) != hipSuccess || A->I.ensure(sizeof(double) * n) != hipSuccess ||
         A->order.ensure(sizeof(int32_t) * n) != hipSuccess || A->key.ensure(sizeof(int32_t) * n) != hipSuccess ||
         A->key_sorted.ensure(sizeof(int32_t) * n) != hipSuccess || A->iota.ensure(sizeof(int32_t) * n) != hipSuccess ||
-        A->seg.ensure(sizeof(double) * kSeg * K * lanes) != hipSuccess ||
+        A->seg.ensure(sizeof(double) * kSeg * (K + 1) * lanes) != hipSuccess ||
         A->partial.ensure(sizeof(double) * 4 * A->grid_b) != hipSuccess ||
         A->counters.ensure(64) != hipSuccess)
         return bail(bhx_fail(ctx, BHX_E_NOMEM, "adjoint buffers"));

@@ -54,7 +54,7 @@ struct Frame {
     // Blandford-Znajek power and the disc efficiency
     int32_t has_jet, pad_;
     double j0, alpha, kappa, dlnj0_da, r_bound;
-    double cone_angle, cone_r0, cone_r1, cone_beta, cone_gamma, cone_gate;
+    double cone_angle, cone_r0, cone_r1, cone_beta, cone_gamma, cone_cos, cone_itaper;
 };
 
 // Integrator state of one ray.
@@ -230,6 +230,10 @@ __host__ __device__ inline T smooth_c2_t(const T& t) {
 }
 template <class T>
 __host__ __device__ inline T abs_t(const T& x) { return (value(x) < 0.0) ? -x : x; }
+// f(x) from its value v and derivative dv at value(x) (sin and cos from one sincos)
+__host__ __device__ inline double with_value(double, double v, double) { return v; }
+template <int N>
+__host__ __device__ inline Dual<N> with_value(const Dual<N>& x, double v, double dv) { return chain(x, v, dv); }
 
 // Integrand of the conical jet at dense-output state (r, theta, p_r) of a
 // ray with constants (a, p_t, p_phi): comoving profile (r0/r)^2 x taper x
@@ -240,13 +244,12 @@ template <class T>
 __host__ __device__ inline T cone_F(const Frame& F, const T& r, const T& th, const T& pr, const T& a, const T& pt,
                                     const T& pph) {
     if (value(r) <= F.cone_r0 || value(r) >= F.cone_r1) return T(0.0);
-    T sn = sin(th);
-    const T cs = cos(th);
-    T acs = abs_t(cs);
-    if (value(acs) > 1.0) acs = T(1.0);
-    const T psi = acos(acs);
-    const T ta = smooth_c2_t((F.cone_angle - psi) / (0.2 * F.cone_angle));
-    if (!(value(ta) > 0.0)) return T(0.0);
+    double snv, csv;
+    sincos_near(value(th), &snv, &csv);
+    if (fabs(csv) <= F.cone_cos) return T(0.0);   // outside the cone: the taper is zero
+    T sn = with_value(th, snv, csv);
+    const T cs = with_value(th, csv, -snv);
+    const T ta = smooth_c2_t((abs_t(cs) - F.cone_cos) * F.cone_itaper);
     const T q = F.cone_r0 / r;
     const T ramp = smooth_c2_t((r - F.cone_r0) / (0.5 * F.cone_r0)) * smooth_c2_t((F.cone_r1 - r) / (0.15 * F.cone_r1));
     if (fabs(value(sn)) < 1e-9) sn = T((value(sn) < 0.0) ? -1e-9 : 1e-9);
@@ -274,8 +277,10 @@ __host__ __device__ inline void cart_hd(double a2, double r, double th, double p
     o[1] = rr * sn * sp;
     o[2] = r * cs;
 }
-__host__ __device__ inline bool near_jet(const Frame& F, const double y[6], const double yn[6]) {
+__host__ __device__ inline bool near_jet(const Frame& F, const Ray& s, const Att& A) {
     if (!F.has_jet) return false;
+    const double* y = s.y;
+    const double* yn = A.yn;
     double p0[3], p1[3];
     cart_hd(F.a2, y[1], y[2], y[3], p0);
     cart_hd(F.a2, yn[1], yn[2], yn[3], p1);
@@ -285,8 +290,10 @@ __host__ __device__ inline bool near_jet(const Frame& F, const double y[6], cons
     if (dd > 0.0) t = dclamp(-(p0[0] * d0 + p0[1] * d1 + p0[2] * d2) / dd, 0.0, 1.0);
     const double q0 = p0[0] + t * d0, q1 = p0[1] + t * d1, q2 = p0[2] + t * d2;
     if (!(sqrt(q0 * q0 + q1 * q1 + q2 * q2) < F.r_bound)) return false;
-    // the sampling gate of the conical jet (z = r cos theta at either end)
-    return fabs(p0[2]) > F.cone_gate * y[1] || fabs(p1[2]) > F.cone_gate * yn[1];
+    // the conical jet's gate: the step's dense output can reach the cone
+    const double c2t = yn[2] - y[2];
+    const double c3t = s.h * s.k1[2] - c2t;
+    return cone_reach(y[2], c2t, c3t, c2t - s.h * A.g7.d[2] - c3t, A.c5[2], F.cone_angle);
 }
 
 // Four-point Gauss-Legendre nodes and weights on [0, 1] (render.cpp:47-50).
@@ -303,7 +310,9 @@ __host__ __device__ inline double gl_w(int q) { return (q == 0 || q == 3) ? 0.17
 __host__ __device__ inline double jet_step(const Frame& F, const double cf[4][5], double h, double s_end,
                                            double pt, double pph) {
     double sum = 0.0;
+#pragma unroll 1
     for (int k = 0; k < kConeSub; ++k)
+#pragma unroll 1
     for (int q = 0; q < 4; ++q) {
         const double u = (k + gl_x(q)) / kConeSub;
         const double sv = u * s_end;
@@ -501,13 +510,17 @@ __host__ __device__ inline void dense_coef_vjp(double h, const double cbar[5][4]
 
 // Adjoint seeds of wk * J_step, J_step = j0 h s_end sum_q w_q F(Y(x_q s_end)),
 // on the step's dense output: node seeds into cbar, constants (a, p_t, p_phi)
-// into cb.  Returns wk dJ_step/ds_end (for an event-truncated step).
+// into cb.  Returns wk dJ_step/ds_end (for an event-truncated step) and
+// J_step in *Js.
 __host__ __device__ inline double jet_seeds(const Frame& F, const double cf[4][5], double h, double s_end,
-                                            double wk, double pt, double pph, double cbar[5][4], double cb[3]) {
+                                            double wk, double pt, double pph, double cbar[5][4], double cb[3],
+                                            double* Js) {
     using D6 = Dual<6>;
     const double coef = wk * F.j0 * h;
-    double dJds = 0.0;
+    double dJds = 0.0, sum = 0.0;
+#pragma unroll 1
     for (int k = 0; k < kConeSub; ++k)
+#pragma unroll 1
     for (int q = 0; q < 4; ++q) {
         const double u = (k + gl_x(q)) / kConeSub;
         const double sv = u * s_end;
@@ -516,6 +529,7 @@ __host__ __device__ inline double jet_seeds(const Frame& F, const double cf[4][5
         const D6 f = cone_F<D6>(F, D6::var(Y[0], 0), D6::var(Y[1], 1), D6::var(Y[2], 2), D6::var(F.a, 3),
                                 D6::var(pt, 4), D6::var(pph, 5));
         if (f.v == 0.0) continue;
+        sum += gl_w(q) * f.v;
         const double wq = coef * s_end * gl_w(q) / kConeSub;
         const double Yb[4] = {wq * f.d[0], wq * f.d[1], wq * f.d[2], 0.0};
         dense_seed(sv, Yb, cbar);
@@ -524,34 +538,60 @@ __host__ __device__ inline double jet_seeds(const Frame& F, const double cf[4][5
         cb[2] += wq * f.d[5];
         dJds += gl_w(q) / kConeSub * (f.v + s_end * u * (f.d[0] * Yp[0] + f.d[1] * Yp[1] + f.d[2] * Yp[2]));
     }
+    *Js = F.j0 * sum * h * s_end / kConeSub;
     return coef * dJds;
 }
 
-// Reverse of the terminal (disc-crossing) step from x with size h and
-// crossing fraction se: the seed Or on r(s*), the jet's sample over [0, s*]
-// with weight wk (when jet), and s* itself, fixed by theta(s*) = pi/2 on the
-// dense output (implicit-function theorem, ds* = -d theta(s*) / theta'(s*)).
-// Adds the adjoints of x to xb and of (a, p_t, p_phi) to cb; returns the
-// jet's sample J_step (0 without jet).
+// Reverse of one frozen step from x with size h: the adjoint xb of its end
+// state (ignored for the terminal step, whose end state is not used) becomes
+// the adjoint of its start; adjoints of (a, p_t, p_phi) accumulate in cb.
+// A step near the jet adds the seeds of wk * its jet sample (over [0, se]
+// for the terminal step, else the whole step) and its sample to *J.  The
+// terminal (disc-crossing) step seeds Or on r(s*), with s* fixed by
+// theta(s*) = pi/2 on the dense output (implicit-function theorem,
+// ds* = -d theta(s*) / theta'(s*)).  One routine for every kind of step, so
+// the kernel holds one copy of the reverse Dormand-Prince step.
+__host__ __device__ inline void step_reverse(const Frame& F, const RayK& K, const double x[4], double h,
+                                             bool jet, bool term, double se, double Or, double wk, double xb[4],
+                                             double cb[3], double* J) {
+    double ynb[4], kb[7][4] = {};
+    for (int c = 0; c < 4; ++c) {
+        ynb[c] = term ? 0.0 : xb[c];
+        xb[c] = 0.0;
+    }
+    const bool dense = term || (jet && wk != 0.0);
+    if (dense) {
+        double cf[4][5], cbar[5][4] = {};
+        step_dense(K, x, h, cf);
+        double Os = 0.0;
+        if (jet && wk != 0.0) {
+            double Js = 0.0;
+            Os = jet_seeds(F, cf, h, term ? se : 1.0, wk, K.pt, K.pph, cbar, cb, &Js);
+            *J += Js;
+        }
+        if (term) {
+            double rv, rd, tv, td;
+            dense_vd(cf[0][0], cf[0][1], cf[0][2], cf[0][3], cf[0][4], se, rv, rd);
+            dense_vd(cf[1][0], cf[1][1], cf[1][2], cf[1][3], cf[1][4], se, tv, td);
+            const double Yb[4] = {Or, -(Or * rd + Os) / td, 0.0, 0.0};
+            dense_seed(se, Yb, cbar);
+        }
+        dense_coef_vjp(h, cbar, xb, ynb, kb);
+    }
+    step_vjp(K, x, h, ynb, kb, dense, xb, cb);
+}
+
+// The terminal step alone (seed Or on r(s*), and the jet over [0, s*] when
+// jet): adds the adjoints of x to xb and of (a, p_t, p_phi) to cb; returns
+// the jet's sample over [0, s*].
 __host__ __device__ inline double terminal_vjp(const Frame& F, const RayK& K, const double x[4], double h,
                                                double se, double Or, bool jet, double wk, double xb[4],
                                                double cb[3]) {
-    double cf[4][5], cbar[5][4] = {};
-    step_dense(K, x, h, cf);
-    double Os = 0.0, Js = 0.0;
-    if (jet) {
-        if (wk != 0.0) Os = jet_seeds(F, cf, h, se, wk, K.pt, K.pph, cbar, cb);
-        Js = jet_step(F, cf, h, se, K.pt, K.pph);
-    }
-    double rv, rd, tv, td;
-    dense_vd(cf[0][0], cf[0][1], cf[0][2], cf[0][3], cf[0][4], se, rv, rd);
-    dense_vd(cf[1][0], cf[1][1], cf[1][2], cf[1][3], cf[1][4], se, tv, td);
-    const double Yb[4] = {Or, -(Or * rd + Os) / td, 0.0, 0.0};
-    dense_seed(se, Yb, cbar);
-    double ynb[4] = {0.0, 0.0, 0.0, 0.0}, kb[7][4] = {};
-    dense_coef_vjp(h, cbar, xb, ynb, kb);
-    step_vjp(K, x, h, ynb, kb, true, xb, cb);
-    return Js;
+    double J = 0.0;
+    double tmp[4] = {0.0, 0.0, 0.0, 0.0};
+    step_reverse(F, K, x, h, jet, true, se, Or, wk, tmp, cb, &J);
+    for (int c = 0; c < 4; ++c) xb[c] += tmp[c];
+    return J;
 }
 
 // ---- one ray, forward ----------------------------------------------------
@@ -637,7 +677,7 @@ __host__ __device__ inline int fwd_step(const Frame& F, Ray& s, double* ck, doub
                 term = BHX_DISC;
             }
         }
-        if (near_jet(F, s.y, A.yn)) {
+        if (near_jet(F, s, A)) {
             double cf[4][5];
             dense4(s, A, cf);
             s.jet += jet_step(F, cf, s.h, s_jet, s.pt, s.pph);
@@ -668,6 +708,7 @@ __host__ __device__ inline int fwd_step(const Frame& F, Ray& s, double* ck, doub
 // ---- one ray, backward ---------------------------------------------------
 
 constexpr int kSeg = 6;   // per stored step: r, theta, p_r, p_theta, h, jet-cull flag
+// A lane's segment buffer holds K + 1 steps: a segment and the terminal step.
 
 // Replay from the checkpoint at ck (the state after n_from accepted steps)
 // until n_to accepted steps: each accepted step's (r, theta, p_r, p_theta, h,
@@ -695,7 +736,7 @@ __host__ __device__ inline void replay(const Frame& F, Ray& s, const double* ck,
             double* o = seg + static_cast<int64_t>(s.nsteps - n_from) * kSeg * stride;
             o[0] = s.y[1]; o[stride] = s.y[2]; o[2 * stride] = s.y[4]; o[3 * stride] = s.y[5];
             o[4 * stride] = s.h;
-            o[5 * stride] = near_jet(F, s.y, A.yn) ? 1.0 : 0.0;
+            o[5 * stride] = near_jet(F, s, A) ? 1.0 : 0.0;
             ++s.nsteps;
             s.tries = 0;
             advance(s, A, fac, lg);
@@ -710,24 +751,6 @@ __host__ __device__ inline bool same_ck(const Ray& s, const double* ck) {
     for (int c = 0; c < 6; ++c)
         if (s.y[c] != ck[c]) return false;
     return s.h == ck[6] && s.pw == ck[7];
-}
-
-// Reverse of one stored step: the adjoint xb of its end state becomes the
-// adjoint of its start; a step near the jet adds its sample's seeds.
-__host__ __device__ inline void bwd_step(const Frame& F, const RayK& K, const double x[4], double h, bool jet,
-                                         double wk, double xb[4], double cb[3]) {
-    double ynb[4] = {xb[0], xb[1], xb[2], xb[3]};
-    double kb[7][4] = {};
-    xb[0] = xb[1] = xb[2] = xb[3] = 0.0;
-    if (jet && wk != 0.0) {
-        double cf[4][5], cbar[5][4] = {};
-        step_dense(K, x, h, cf);
-        jet_seeds(F, cf, h, 1.0, wk, K.pt, K.pph, cbar, cb);
-        dense_coef_vjp(h, cbar, xb, ynb, kb);
-        step_vjp(K, x, h, ynb, kb, true, xb, cb);
-    } else {
-        step_vjp(K, x, h, ynb, kb, false, xb, cb);
-    }
 }
 
 // Adjoint of w * I for one ray that ended after n_acc accepted steps with
@@ -752,16 +775,16 @@ __host__ __device__ inline int bwd_ray(const Frame& F, const RaygenArgs& G, int6
     const bool disc = cls == BHX_DISC;
     double xb[4] = {0.0, 0.0, 0.0, 0.0};
     double cb[3] = {0.0, 0.0, 0.0};
-    double I_disc = 0.0, J = 0.0;
-    // the last segment: replayed up to the terminal step (a disc hit) or to
-    // the end (every other class: all steps are full steps)
+    double I_disc = 0.0, J = 0.0, se = 0.5, Or = 0.0;
+    // the last segment: replayed up to the terminal step (a disc hit: stored
+    // after the segment's steps) or to the end (every other class: all steps
+    // are full steps)
     const int n_reg = disc ? n_acc - 1 : n_acc;
     const int j_last = (n_reg == 0) ? 0 : (n_reg - 1) / F.K;
     const int last_from = j_last * F.K;
     replay(F, s, ck + j_last * kCk, last_from, n_reg, seg, stride);
     if (disc) {
-        const double xt[4] = {s.y[1], s.y[2], s.y[4], s.y[5]};
-        double se = 0.5, r_hit = 0.0;
+        double r_hit = 0.0;
         bool jet_term = false;
         for (;;) {
             clamp_h(F, s);
@@ -776,26 +799,29 @@ __host__ __device__ inline int bwd_ray(const Frame& F, const RaygenArgs& G, int6
                 se = root_theta(ct[0], ct[1], ct[2], ct[3], ct[4], s.c_prev, s.y[2], A.yn[2]);
                 r_hit = dense_comp(cr[0], cr[1], cr[2], cr[3], cr[4], se);
                 if (!(s.c_prev * A.g7.cs < 0.0) || !(r_hit >= F.r_in && r_hit <= F.r_out)) ++mismatch;
-                jet_term = near_jet(F, s.y, A.yn);
+                jet_term = near_jet(F, s, A);
                 break;
             }
             s.h *= dmin(fac, 1.0);
             if (++s.tries > 40) { ++mismatch; return mismatch; }
         }
+        double* o = seg + static_cast<int64_t>(n_reg - last_from) * kSeg * stride;
+        o[0] = s.y[1]; o[stride] = s.y[2]; o[2 * stride] = s.y[4]; o[3 * stride] = s.y[5];
+        o[4 * stride] = s.h;
+        o[5 * stride] = jet_term ? 1.0 : 0.0;
         I_disc = disc_intensity(F, r_hit, s.pt, s.pph);
-        double Or = 0.0;
         if (I_disc > 0.0) {
             double dI[4];
             disc_intensity_grad(F, r_hit, s.pt, s.pph, dI);
             Or = w * dI[0];
             cb[0] += w * dI[1]; cb[1] += w * dI[2]; cb[2] += w * dI[3];
         }
-        J += terminal_vjp(F, K, xt, s.h, se, Or, jet_term, wk, xb, cb);
     }
-    // sweep back through the stored segments
+    // sweep back through the stored segments (the terminal step first)
+    const int n_last = disc ? n_reg + 1 : n_reg;
     for (int j = j_last; j >= 0; --j) {
         const int n_from = j * F.K;
-        const int n_to = (j == j_last) ? n_reg : n_from + F.K;
+        const int n_to = (j == j_last) ? n_last : n_from + F.K;
         if (j != j_last) {
             replay(F, s, ck + j * kCk, n_from, n_to, seg, stride);
             if (!same_ck(s, ck + (j + 1) * kCk)) ++mismatch;
@@ -803,13 +829,8 @@ __host__ __device__ inline int bwd_ray(const Frame& F, const RaygenArgs& G, int6
         for (int i = n_to - n_from - 1; i >= 0; --i) {
             const double* o = seg + static_cast<int64_t>(i) * kSeg * stride;
             const double xi[4] = {o[0], o[stride], o[2 * stride], o[3 * stride]};
-            const bool jet = o[5 * stride] != 0.0;
-            if (jet) {
-                double cf[4][5];
-                step_dense(K, xi, o[4 * stride], cf);
-                J += jet_step(F, cf, o[4 * stride], 1.0, s.pt, s.pph);
-            }
-            bwd_step(F, K, xi, o[4 * stride], jet, wk, xb, cb);
+            const bool term = disc && j == j_last && i == n_to - n_from - 1;
+            step_reverse(F, K, xi, o[4 * stride], o[5 * stride] != 0.0, term, se, Or, wk, xb, cb, &J);
         }
     }
     // camera: x0 = (r_cam, theta_cam, p_r, p_theta) and (p_t, p_phi) as

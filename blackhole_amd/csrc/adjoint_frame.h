@@ -85,7 +85,7 @@ inline bool make_frame(const bhx_scene& s, int K, Frame& F, RaygenArgs& G) {
         F.alpha = s.jet_alpha;
         F.r_bound = s.jet_r_bound;
         F.cone_angle = E.cone_angle; F.cone_r0 = E.cone_r0; F.cone_r1 = E.cone_r1;
-        F.cone_beta = E.cone_beta; F.cone_gamma = E.cone_gamma; F.cone_gate = E.cone_gate;
+        F.cone_beta = E.cone_beta; F.cone_gamma = E.cone_gamma; F.cone_cos = E.cone_cos; F.cone_itaper = E.cone_itaper;
         const double nu_0 = 299792458.0 / 550e-9, nu_lo = 1.0e9, nu_hi = 1.0e17;
         const double p = 1.0 - s.jet_alpha;
         const double band = (std::fabs(p) > 1e-9) ? (std::pow(nu_hi, p) - std::pow(nu_lo, p)) / p

@@ -27,8 +27,9 @@ int check_scene(bhx_context* ctx, const bhx_scene* s) {
 
 // Centre-first order of 8 x 8 tiles: the photon ring (longest rays) sits near
 // the optical axis, so starting there shortens the queue's tail.
-int ensure_tiles(bhx_context* ctx, int W, int H, int S) {
-    const auto key = std::make_tuple(W, H, S);
+int ensure_tiles(bhx_context* ctx, int W, int H, int S, bool band, double roll) {
+    if (!band) roll = 0.0;
+    const auto key = std::make_tuple(W, H, S, band, roll);
     auto it = ctx->tiles.find(key);
     if (it != ctx->tiles.end()) {
         ctx->cur_tiles = &it->second;
@@ -43,7 +44,7 @@ int ensure_tiles(bhx_context* ctx, int W, int H, int S) {
         ctx->tiles.clear();
     }
     std::vector<int32_t> order, start;
-    host::tile_queue(W, H, S, 0, order, start);
+    host::tile_queue(W, H, S, 0, order, start, band, roll);
     const int n = static_cast<int>(order.size());
     bhx_context::TileSet& t = ctx->tiles[key];
     HIPCHK(t.order.ensure(sizeof(int32_t) * n));
@@ -212,7 +213,7 @@ int bhx_trace_dev(bhx_context* ctx, const bhx_scene* s, const bhx_rays* rays, bh
     HIPCHK(hipSetDevice(ctx->device));
     hipStream_t st = static_cast<hipStream_t>(stream);
     const int Sspp = std::max(1, s->sqrt_spp);
-    rc = ensure_tiles(ctx, s->width, s->height, Sspp);
+    rc = ensure_tiles(ctx, s->width, s->height, Sspp, host::axis_band(*s), s->cam_roll);
     if (rc) return rc;
     const int64_t n_rays = static_cast<int64_t>(s->width) * s->height * Sspp * Sspp;
     HIPCHK(ctx->init.ensure(sizeof(double) * kInitRows * n_rays));

@@ -177,7 +177,8 @@ int bhx_batch_create(bhx_context* ctx, const bhx_scene* scenes, int32_t n, bhx_b
     for (int i : b->queue_order) {
         const bhx_scene& s = scenes[i];
         std::vector<int32_t> order, start;
-        host::tile_queue(s.width, s.height, std::max(1, s.sqrt_spp), q, order, start);
+        host::tile_queue(s.width, s.height, std::max(1, s.sqrt_spp), q, order, start, host::axis_band(s),
+                         s.cam_roll);
         toff[i] = order_all.size();
         soff[i] = start_all.size();
         ntiles[i] = static_cast<int>(order.size());

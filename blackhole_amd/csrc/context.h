@@ -77,7 +77,7 @@ struct bhx_context {
         DevBuf order, start;
         int tiles_x = 0, n_tiles = 0;
     };
-    std::map<std::tuple<int, int, int>, TileSet> tiles;
+    std::map<std::tuple<int, int, int, bool, double>, TileSet> tiles;   // (W, H, S, axis band, roll)
     const TileSet* cur_tiles = nullptr;
     // initial ray states in queue order (kInitRows x n) and slot -> ray map
     DevBuf init, q2ray;

@@ -86,15 +86,16 @@ __device__ inline double jet_boost(const EventParams& J, double a, double a2, do
 
 // Conical jet (model 1, no counterpart in the reference): comoving emissivity
 // profile at (r, cos theta) - (r0 / r)^2 inside two cones of half opening
-// cone_angle about the spin axis, tapered over the outer fifth of the opening
-// and ramped at both ends (the end ramps of the reference's jets,
-// jet.h:199-202) with the quintic step, so that the profile has a continuous
-// second derivative and its path integral a smooth gradient.
+// cone_angle about the spin axis, tapered in |cos theta| between the cone's
+// edge and 0.8 of its opening and ramped at both ends (the end ramps of the
+// reference's jets, jet.h:199-202), all with the quintic step, so that the
+// profile has a continuous second derivative and its path integral a smooth
+// gradient.
 __device__ inline double cone_shape(const EventParams& J, double r, double cs) {
     if (r <= J.cone_r0 || r >= J.cone_r1) return 0.0;
-    const double psi = acos(dmin(fabs(cs), 1.0));
-    const double ta = smooth_c2((J.cone_angle - psi) / (0.2 * J.cone_angle));
-    if (ta <= 0.0) return 0.0;
+    const double mu = fabs(cs);
+    if (mu <= J.cone_cos) return 0.0;   // outside the cone
+    const double ta = smooth_c2((mu - J.cone_cos) * J.cone_itaper);
     const double q = J.cone_r0 / r;
     const double ramp = smooth_c2((r - J.cone_r0) / (0.5 * J.cone_r0)) *
                         smooth_c2((J.cone_r1 - r) / (0.15 * J.cone_r1));
@@ -102,24 +103,22 @@ __device__ inline double cone_shape(const EventParams& J, double r, double cs) {
 }
 
 // g^(2+alpha) of the conical jet's plasma, radial outflow u = Gamma (e_t +
-// beta e_r) on the ZAMO tetrad, seen along momentum p.  0 when g <= 0.
-__device__ inline double cone_boost(const EventParams& J, double a, double a2, double r, double th,
+// beta e_r) on the ZAMO tetrad, seen along momentum p, at (r, theta) with
+// (sin, cos) theta = (sn, cs).  0 when g <= 0.
+__device__ inline double cone_boost(const EventParams& J, double a, double a2, double r, double sn, double cs,
                                     double pt, double pr, double pph) {
-    double sn, cs;
-    sincos_near(th, &sn, &cs);
     if (fabs(sn) < 1e-9) sn = (sn < 0.0) ? -1e-9 : 1e-9;
     const double sn2 = sn * sn;
     const double S = r * r + a2 * cs * cs;
     const double D = r * r - 2.0 * r + a2;
     const double A = (r * r + a2) * (r * r + a2) - a2 * D * sn2;
-    const double lapse = sqrt(S * D / A);
-    const double omega = 2.0 * a * r / A;
-    const double u0 = J.cone_gamma / lapse;
-    const double u1 = J.cone_gamma * J.cone_beta * sqrt(D / S);
-    const double u3 = J.cone_gamma * omega / lapse;
-    const double ku = pt * u0 + pr * u1 + pph * u3;
+    const double iA = rcp(A);
+    const double ilapse = rcp(sqrt(S * D * iA));
+    const double omega = 2.0 * a * r * iA;
+    // k.u = Gamma ((p_t + omega p_phi) / lapse + beta p_r sqrt(D / S))
+    const double ku = J.cone_gamma * ((pt + omega * pph) * ilapse + J.cone_beta * pr * sqrt(D * rcp(S)));
     if (!(ku > 1e-12)) return 0.0;
-    return pow_pos(1.0 / ku, 2.0 + J.alpha);
+    return pow_pos(rcp(ku), 2.0 + J.alpha);
 }
 
 // bl_to_cartesian (geodesic.h:256-262).

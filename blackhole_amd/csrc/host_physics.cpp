@@ -264,14 +264,15 @@ int bhx_setup_conical_jet(bhx_scene* s, double phi_flux, double eps_rad) {
     const double L_rad = eps_rad * P_jet;
     // emitting volume of both cones, weighted by the profile (r0/r)^2 x taper x ramps
     const double th_j = s->jet_cone_angle, r0 = s->jet_cone_r0, r1 = s->jet_cone_r1;
+    // angular part: integral of the taper over sin psi d psi = d(cos psi),
+    // the taper being smooth_c2((mu - cos th_j) / (cos 0.8 th_j - cos th_j))
     double ang = 0.0;
     {
         const int N = 400;
-        const double dpsi = th_j / N;
-        for (int i = 0; i < N; ++i) {
-            const double psi = (i + 0.5) * dpsi;
-            ang += smooth_c2((th_j - psi) / (0.2 * th_j)) * std::sin(psi) * dpsi;
-        }
+        const double c0 = std::cos(th_j), c1 = std::cos(0.8 * th_j);
+        ang = 1.0 - c1;   // mu in [cos 0.8 th_j, 1]: taper 1
+        const double dmu = (c1 - c0) / N;
+        for (int i = 0; i < N; ++i) ang += smooth_c2((i + 0.5) / N) * dmu;
     }
     double rad = 0.0;
     {

@@ -271,20 +271,26 @@ EventParams event_params(const bhx_scene& s) {
     E.cone_r1 = s.jet_cone_r1;
     E.cone_gamma = std::max(1.0, s.jet_gamma_max);
     E.cone_beta = std::sqrt(std::max(0.0, 1.0 - 1.0 / (E.cone_gamma * E.cone_gamma)));
-    E.cone_gate = std::cos(std::min(s.jet_cone_angle + kConeGateMargin, 1.5707963267948966));
+    E.cone_cos = std::cos(s.jet_cone_angle);
+    E.cone_itaper = 1.0 / (std::cos(0.8 * s.jet_cone_angle) - E.cone_cos);
     return E;
 }
 
 void tile_queue(int W, int H, int S, int32_t q_base, std::vector<int32_t>& order,
-                std::vector<int32_t>& start) {
+                std::vector<int32_t>& start, bool axis_band, double roll) {
     const int tw = (W + 7) / 8, th = (H + 7) / 8;
     const int n = tw * th;
     order.resize(n);
     std::vector<double> key(n);
+    const double cr = std::cos(roll), sr = std::sin(roll);
     for (int i = 0; i < n; ++i) {
         order[i] = i;
         const double cx = (i % tw) * 8 + 4 - 0.5 * W, cy = (i / tw) * 8 + 4 - 0.5 * H;
         key[i] = cx * cx + cy * cy;
+        if (axis_band) {   // the camera's x (after roll) vanishes on the axis' image
+            const double perp = cr * cx + sr * cy;
+            key[i] = std::min(key[i], perp * perp);
+        }
     }
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
     start.assign(n + 1, q_base);

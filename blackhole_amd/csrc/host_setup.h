@@ -42,9 +42,14 @@ ShadeArgs shade_args(const bhx_scene& s);
 // Work-queue order of a W x H frame with S x S samples per pixel: 8 x 8-pixel
 // tiles sorted by distance of their centre from the image centre (the photon
 // ring's long rays start first), and the first queue slot of each queue tile
-// (n_tiles + 1 entries, slot 0 = q_base).
+// (n_tiles + 1 entries, slot 0 = q_base).  With axis_band, the distance is
+// the smaller of that and the distance from the spin axis' image (the line
+// through the centre that the camera's roll turns): rays along the axis of a
+// conical jet take the most steps inside the jet, so they start first too.
 void tile_queue(int W, int H, int S, int32_t q_base, std::vector<int32_t>& order,
-                std::vector<int32_t>& start);
+                std::vector<int32_t>& start, bool axis_band = false, double roll = 0.0);
+// Whether a scene's queue puts the spin axis' image first (the conical jet).
+inline bool axis_band(const bhx_scene& s) { return s.jet_enabled && s.jet_model == 1; }
 
 // Kerr helpers used by the set-up code.
 double isco_radius(double a, int sense);

@@ -1,7 +1,14 @@
 // params.h - launch arguments shared by the host API and the HIP kernels.
 #pragma once
 
+#include <math.h>
 #include <stdint.h>
+
+#ifdef __HIPCC__
+#define BHX_HD __host__ __device__
+#else
+#define BHX_HD
+#endif
 
 namespace bhx {
 
@@ -12,18 +19,25 @@ struct EventParams {
     double zeta, Omega_F, j_nu0, r_bound, gamma_zb;   // gamma_zb = Jet::gamma_at(z_base)
     double sun_c[3], sun_R;
     // conical jet (model 1): half opening, radial extent, outflow speed and
-    // Lorentz factor; cone_gate = cos(min(cone_angle + kConeGateMargin, pi/2))
-    double cone_angle, cone_r0, cone_r1, cone_beta, cone_gamma, cone_gate;
+    // Lorentz factor; cone_cos = cos(cone_angle), cone_itaper =
+    // 1 / (cos(0.8 cone_angle) - cos(cone_angle))
+    double cone_angle, cone_r0, cone_r1, cone_beta, cone_gamma, cone_cos, cone_itaper;
     int32_t model, pad;
 };
 
 // Sampling of the conical jet along a step: kConeSub sub-intervals of
-// 4-point Gauss-Legendre each, on steps with an end within cone_angle +
-// kConeGateMargin [rad] of the spin axis.  A step moves the polar angle by
-// less than 0.1 rad (h <= 0.05 (r - r_h) + 0.01), so a step with both ends
-// outside the gate lies outside the cone, where the emission is zero.
+// 4-point Gauss-Legendre each, on the steps whose dense output can reach the
+// cone.  With theta(s) = c1 + s c2 + s(1-s) c3 + s^2(1-s) c4 + s^2(1-s)^2 c5,
+// |theta(s) - theta(0)| <= cone_reach = |c2| + |c3|/4 + 4|c4|/27 + |c5|/16 on
+// [0, 1]; a step whose start lies further than cone_angle + cone_reach from
+// the axis stays outside the cone, where the emission is zero.
 constexpr int kConeSub = 8;
-constexpr double kConeGateMargin = 0.2;
+BHX_HD inline bool cone_reach(double th0, double c2, double c3, double c4, double c5,
+                                           double cone_angle) {
+    const double reach = fabs(c2) + 0.25 * fabs(c3) + (4.0 / 27.0) * fabs(c4) + 0.0625 * fabs(c5);
+    const double psi0 = th0 < 3.14159265358979323846 - th0 ? th0 : 3.14159265358979323846 - th0;
+    return psi0 - reach < cone_angle;
+}
 
 // One scene of a batched launch: the tracer's per-scene constants (the
 // uniform fields of TraceArgs) plus its event parameters.  The batched tracer

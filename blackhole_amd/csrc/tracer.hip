@@ -44,6 +44,14 @@ __constant__ const double kGLx[4] = {0.06943184420297371, 0.33000947820757187, 0
 __constant__ const double kGLw[4] = {0.17392742256872693, 0.32607257743127307, 0.32607257743127307,
                                      0.17392742256872693};
 
+// A wave-uniform double moved into scalar registers.
+__device__ __forceinline__ double uniform_d(double x) {
+    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(x));
+    const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(b));
+    const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(b >> 32));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
 // ---------------------------------------------------------------------------
 // Ray generation: one block of 64 threads per queue tile.
 // ---------------------------------------------------------------------------
@@ -117,6 +125,25 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
     int nsteps = 0, natt = 0, tries = 0;
 #pragma unroll
     for (int c = 0; c < 6; ++c) { y[c] = 0.0; k1[c] = 0.0; }
+
+    // ---- per-step event constants ------------------------------------------
+    // The single-scene kernel reads the jet cull and the Sun test's constants
+    // once, into scalar registers: read inside the step loop, each would
+    // expose a full memory latency on every step.  The batched kernel reads
+    // them from its LDS table.
+    double ev_r_bound = 0.0, ev_cone_angle = 0.0, ev_sun_R = 0.0, ev_sun_c[3] = {0.0, 0.0, 0.0};
+    int ev_model = 0;
+    if constexpr (!kBatch) {
+        if (P.has_jet || P.has_sun) {
+            ev_r_bound = uniform_d(P.ev->r_bound);
+            ev_cone_angle = uniform_d(P.ev->cone_angle);
+            ev_sun_R = uniform_d(P.ev->sun_R);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) ev_sun_c[c] = uniform_d(P.ev->sun_c[c]);
+            ev_model = __builtin_amdgcn_readfirstlane(P.ev->model);
+        }
+    }
+#define EVS(f) (kBatch ? sk[sid].ev.f : ev_##f)
 
     // ---- wave-uniform queue state ------------------------------------------
     int c_next = 0, c_end = 0;
@@ -281,17 +308,18 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
             if (SK(has_jet) || SK(has_sun)) {
                 to_cart(la2, y[1], y[2], y[3], p0);
                 to_cart(la2, yn[1], yn[2], yn[3], p1);
-                if (SK(has_jet)) near_jet = seg_dist(p0, p1) < EV.r_bound;          // render.cpp:151-152
-                // conical jet: only steps with an end inside the sampling gate
-                // (z = r cos theta) can reach the cone
-                if (near_jet && EV.model == 1)
-                    near_jet = fabs(p0[2]) > EV.cone_gate * y[1] || fabs(p1[2]) > EV.cone_gate * yn[1];
+                if (SK(has_jet)) near_jet = seg_dist(p0, p1) < EVS(r_bound);        // render.cpp:151-152
+                // conical jet: only steps whose dense output can reach the cone
+                if (near_jet && EVS(model) == 1) {
+                    const double c2t = yn[2] - y[2];
+                    const double c3t = h * k1[2] - c2t;
+                    near_jet = cone_reach(y[2], c2t, c3t, c2t - h * g7.d[2] - c3t, c5[2], EVS(cone_angle));
+                }
                 if (SK(has_sun) && !ns_hit && !horizon) {                             // render.cpp:203-209
-                    const double* C = EV.sun_c;
                     const double e0 = p1[0] - p0[0], e1 = p1[1] - p0[1], e2 = p1[2] - p0[2];
                     const double seg_len = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
-                    const double f0 = p0[0] - C[0], f1 = p0[1] - C[1], f2 = p0[2] - C[2];
-                    sun_cand = sqrt(f0 * f0 + f1 * f1 + f2 * f2) <= seg_len + EV.sun_R * 1.5;
+                    const double f0 = p0[0] - EVS(sun_c)[0], f1 = p0[1] - EVS(sun_c)[1], f2 = p0[2] - EVS(sun_c)[2];
+                    sun_cand = sqrt(f0 * f0 + f1 * f1 + f2 * f2) <= seg_len + EVS(sun_R) * 1.5;
                 }
             }
             double s_jet = -1.0;   // fraction of the step the jet integral covers
@@ -368,13 +396,13 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
                     }
                 }
                 if (term < 0) s_jet = 1.0;   // nothing stopped the ray: the whole step counts
-                if (near_jet && s_jet > 0.0 && EV.model == 1) {
+                if (near_jet && s_jet > 0.0 && EVS(model) == 1) {
                     // ---- conical jet: kConeSub x 4-point Gauss-Legendre
                     const EventParams& J = EV;
                     double sum = 0.0;
 #pragma unroll 1
                     for (int ks = 0; ks < kConeSub; ++ks) {
-#pragma unroll
+#pragma unroll 1
                         for (int qd = 0; qd < 4; ++qd) {
                             const double sv = (ks + kGLx[qd]) * (1.0 / kConeSub) * s_jet;
                             const double rs = dense_comp(c1[1], c2[1], c3[1], c4[1], c5[1], sv);
@@ -384,7 +412,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
                             const double sh = cone_shape(J, rs, cs);
                             if (sh <= 0.0) continue;
                             const double prs = dense_comp(c1[4], c2[4], c3[4], c4[4], c5[4], sv);
-                            const double b = cone_boost(J, la, la2, rs, ths, pt, prs, pph);
+                            const double b = cone_boost(J, la, la2, rs, sn, cs, pt, prs, pph);
                             if (b <= 0.0) continue;
                             sum += kGLw[qd] * sh * b;
                         }
@@ -394,7 +422,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
                     // ---- jet emission: 4-point Gauss-Legendre (render.cpp:107-122)
                     const EventParams& J = EV;
                     double sum = 0.0;
-#pragma unroll
+#pragma unroll 1
                     for (int qd = 0; qd < 4; ++qd) {
                         const double sv = kGLx[qd] * s_jet;
                         const double rs = dense_comp(c1[1], c2[1], c3[1], c4[1], c5[1], sv);
@@ -465,6 +493,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
 }
 #undef SK
 #undef EV
+#undef EVS
 
 __device__ __forceinline__ void flush_totals(const TraceArgs& P, unsigned long long* blk) {
     // frame totals: one global atomic per block

@@ -669,9 +669,11 @@ struct JetModel {
     // conical jet (bhx_scene.jet_model = 1), the build's own model: no
     // reference counterpart; the same formulas as the device's cone_shape /
     // cone_boost, written with the C library's functions.  Profile (r0/r)^2
-    // x taper x end ramps, all three with the quintic step; sampled on steps
-    // with an end within cone_angle + 0.2 rad of the axis by 8 sub-intervals
-    // of 4-point Gauss-Legendre.
+    // x taper x end ramps, all three with the quintic step (the taper in
+    // |cos theta|, from the cone's edge to 0.8 of its opening); sampled by 8
+    // sub-intervals of 4-point Gauss-Legendre on the steps whose dense output
+    // can reach the cone: |theta(s) - theta(0)| <= |c2| + |c3|/4 + 4|c4|/27 +
+    // |c5|/16 over the step.
     int model = 0;
     double cone_angle = 0.0, cone_r0 = 0.0, cone_r1 = 0.0;
     static double smooth_c2(double t) {
@@ -680,17 +682,19 @@ struct JetModel {
     }
     double cone_shape(double r, double cs) const {
         if (r <= cone_r0 || r >= cone_r1) return 0.0;
-        const double psi = std::acos(std::min(std::fabs(cs), 1.0));
-        const double ta = smooth_c2((cone_angle - psi) / (0.2 * cone_angle));
-        if (ta <= 0.0) return 0.0;
+        const double c0 = std::cos(cone_angle), c1 = std::cos(0.8 * cone_angle);
+        const double mu = std::fabs(cs);
+        if (mu <= c0) return 0.0;   // outside the cone
+        const double ta = smooth_c2((mu - c0) / (c1 - c0));
         const double q = cone_r0 / r;
         const double ramp = smooth_c2((r - cone_r0) / (0.5 * cone_r0)) * smooth_c2((cone_r1 - r) / (0.15 * cone_r1));
         return q * q * ta * ramp;
     }
-    // a step from cartesian p0 (radius r0_) to p1 (radius r1_) can reach the cone
-    bool cone_gate(const double p0[3], double r0_, const double p1[3], double r1_) const {
-        const double g = std::cos(std::min(cone_angle + 0.2, 1.5707963267948966));
-        return std::fabs(p0[2]) > g * r0_ || std::fabs(p1[2]) > g * r1_;
+    // the step's dense output (theta coefficients c1..c5) can reach the cone
+    bool cone_gate(double c1, double c2, double c3, double c4, double c5) const {
+        const double reach = std::fabs(c2) + std::fabs(c3) / 4.0 + 4.0 * std::fabs(c4) / 27.0 + std::fabs(c5) / 16.0;
+        const double psi0 = std::min(c1, 3.14159265358979323846 - c1);
+        return psi0 - reach < cone_angle;
     }
     void cone_velocity(const Pt& p, double u[4]) const {
         const double G = std::max(1.0, gamma_max);
@@ -1001,7 +1005,8 @@ void trace_state(const Scene& sc, const double y0[8], RayResult& out) {
         double p1[3];
         to_cartesian(a, yn[YR], yn[YH], yn[YP], p1);
         bool near_jet = c.jet_enabled && seg_dist_origin(p0, p1) < c.jet_r_bound;
-        if (near_jet && sc.jet.model == 1) near_jet = sc.jet.cone_gate(p0, y[YR], p1, yn[YR]);
+        if (near_jet && sc.jet.model == 1)
+            near_jet = sc.jet.cone_gate(seg.c1[YH], seg.c2[YH], seg.c3[YH], seg.c4[YH], seg.c5[YH]);
 
         if (c.ns_enabled && yn[YR] <= c.ns_R) {  // render.cpp:158-169
             double lo = 0.0, hi = 1.0;

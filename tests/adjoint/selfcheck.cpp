@@ -217,7 +217,7 @@ static void check_ray(const char* name, const bhx_scene& sc, int K, int64_t stri
         std::printf("%s 0 1 mismatch 0\n", name);
         return;
     }
-    std::vector<double> ck(static_cast<size_t>(F.cap) * kCk), seg(static_cast<size_t>(K) * kSeg);
+    std::vector<double> ck(static_cast<size_t>(F.cap) * kCk), seg(static_cast<size_t>(K + 1) * kSeg);
     double worst = 0.0;
     int n = 0, mism = 0, n_jet = 0;
     for (int64_t ray = 0; ray < (int64_t)sc.width * sc.height && n < n_max; ray += stride) {

@@ -67,7 +67,9 @@ def test_radiated_power_identity():
     """4 pi j_nu0 nu0^alpha band V r_g^2 = eps_rad P_BZ, V by scipy."""
     sc = _cone()
     th, r0, r1 = sc.jet_cone_angle, sc.jet_cone_r0, sc.jet_cone_r1
-    ang, _ = integrate.quad(lambda p: _smooth_c2((th - p) / (0.2 * th)) * math.sin(p), 0.0, th,
+    # the taper in mu = |cos theta|: quintic step from cos(th) (edge) to cos(0.8 th)
+    c0, c1 = math.cos(th), math.cos(0.8 * th)
+    ang, _ = integrate.quad(lambda p: _smooth_c2((math.cos(p) - c0) / (c1 - c0)) * math.sin(p), 0.0, th,
                             points=[0.8 * th], epsabs=0.0, epsrel=1e-12, limit=200)
     rad, _ = integrate.quad(lambda r: (r0 / r) ** 2 * _smooth_c2((r - r0) / (0.5 * r0))
                             * _smooth_c2((r1 - r) / (0.15 * r1)) * r * r, r0, r1,
