@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libbhx.so")
 
-HIP_SOURCES = ["tracer.hip", "shade.hip", "api.hip", "batch.hip", "adjoint.hip"]
+HIP_SOURCES = ["tracer.hip", "shade.hip", "api.hip", "batch.hip", "adjoint.hip", "display.hip"]
 # Per-file flags.  The tracer's loop uses ~70 distinct FP64 constants (DP5
 # tableau, sincos/log/exp coefficients); FP64 VALU ops take no 64-bit
 # literals, so machine LICM would pin every one of them in two registers for
@@ -21,8 +21,10 @@ HIP_SOURCES = ["tracer.hip", "shade.hip", "api.hip", "batch.hip", "adjoint.hip"]
 # DP5 attempt into two kernels and must round identically, so FMAs are formed
 # per source expression only (-ffp-contract=on), never across expressions by
 # the backend, where the choice depends on the surrounding kernel.
+# display.hip: the glare's sums must be the reference's IEEE sequence (no FMA).
 EXTRA = {"tracer.hip": ["-mllvm", "-disable-machine-licm"],
-         "adjoint.hip": ["-mllvm", "-disable-machine-licm", "-ffp-contract=on"]}
+         "adjoint.hip": ["-mllvm", "-disable-machine-licm", "-ffp-contract=on"],
+         "display.hip": ["-ffp-contract=off"]}
 CXX_SOURCES = ["host_setup.cpp", "host_physics.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

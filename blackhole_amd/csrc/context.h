@@ -59,7 +59,7 @@ struct bhx_context {
     int device = 0;
     int n_cu = 0;
     std::string err;
-    bhx_launch launch{256, 0, 64, 16};
+    bhx_launch launch{256, 0, 32, 8};   // chunk 32, refill 8: fastest of a sweep on the 1024² frame
     // tables
     double* d_bb = nullptr;
     // catalogue (cell CSR + star rows)
@@ -90,6 +90,8 @@ struct bhx_context {
     DevBuf flux_partial, lum_hist, lum_npos;
     // frame buffers of bhx_render
     DevBuf r_cls, r_y, r_jet, r_xyz;
+    // veiling glare: row pass, the two blurred images, the taps
+    DevBuf glare_tmp, glare_near, glare_far, glare_taps;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 

@@ -153,6 +153,17 @@ class Tracer:
                                                               self._stream(stream)), "bhx_auto_exposure_dev")
         return e.value
 
+    def add_glare(self, xyz, strength, radius_px=3.0, stream=None):
+        """img::add_glare (src/image.cpp:47-106) in place on a contiguous
+        (H, W, 3) float64 device tensor; returns it."""
+        if xyz.dim() != 3 or xyz.shape[2] != 3 or not xyz.is_contiguous():
+            raise ValueError("add_glare needs a contiguous (H, W, 3) tensor")
+        H, W = int(xyz.shape[0]), int(xyz.shape[1])
+        native.check(self.ctx, self.lib.bhx_add_glare_dev(self.ctx, _ptr(xyz), W, H, float(strength),
+                                                          float(radius_px), self._stream(stream)),
+                     "bhx_add_glare_dev")
+        return xyz
+
     def develop(self, xyz, exposure, tonemap=TM_HDR, log_decades=16.0, stream=None):
         t = self.torch
         n = xyz.numel() // 3
@@ -211,6 +222,14 @@ class Batch:
         native.check(self.tr.ctx, self.lib.bhx_batch_stats(self.b, st, C.byref(ms), self.tr._stream(stream)),
                      "bhx_batch_stats")
         return list(st), ms.value
+
+    def xyz_ptr(self, i):
+        """Device address of scene i's (H, W, 3) float64 XYZ pixels after a run
+        (valid until the next run or close)."""
+        p = C.c_void_p()
+        native.check(self.tr.ctx, self.lib.bhx_batch_scene(self.b, i, None, None, None, C.byref(p)),
+                     "bhx_batch_scene")
+        return p
 
     def scene(self, i, stream=None):
         """Host copies of scene i's results: cls, y (8, n), jet, steps, attempts, xyz (H, W, 3)."""

@@ -194,6 +194,15 @@ int bhx_auto_exposure_dev(bhx_context *ctx, const double *xyz, int64_t n_pixels,
                           double *exposure, void *stream);
 
 /* img::develop (src/image.cpp:119-191): device XYZ frame -> device sRGB8. */
+/* img::add_glare (src/image.cpp:47-106) on a device XYZ image (row-major,
+ * width x height x 3 doubles), in place: the image plus a sigma = radius_px
+ * and a sigma = 3 radius_px Gaussian of itself, weights 1 - strength,
+ * 0.7 strength, 0.3 strength.  Bit-identical to the reference's CPU result.
+ * No-op when strength <= 0 or radius_px <= 0; BHX_E_TOO_LARGE above a
+ * radius of 113 px.  Stream-ordered (one host synchronisation of `stream`
+ * before its taps are uploaded). */
+int bhx_add_glare_dev(bhx_context *ctx, double *xyz, int32_t width, int32_t height, double strength,
+                      double radius_px, void *stream);
 int bhx_develop_dev(bhx_context *ctx, const double *xyz, int64_t n_pixels, double exposure,
                     int32_t tonemap, double log_decades, uint8_t *rgb, void *stream);
 

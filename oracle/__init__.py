@@ -65,6 +65,7 @@ def lib():
         _lib.oracle_page_thorne_W.argtypes = [D, D, D]
         _lib.oracle_auto_exposure.argtypes = [P, C.c_int64, D]
         _lib.oracle_develop.argtypes = [P, C.c_int64, D, C.c_int, D, P]
+        _lib.oracle_add_glare.argtypes = [P, C.c_int32, C.c_int32, D, D]
         _lib.oracle_blackbody_xyz.argtypes = [D, P]
         _lib.oracle_blackbody_xyz_exact.argtypes = [D, P]
         _lib.oracle_powerlaw_xyz.argtypes = [D, P]
@@ -90,6 +91,7 @@ def ref():
         _ref.bhref_auto_exposure.restype = C.c_double
         _ref.bhref_auto_exposure.argtypes = [P, C.c_int64, D]
         _ref.bhref_develop.argtypes = [P, C.c_int64, D, C.c_int, D, P]
+        _ref.bhref_add_glare.argtypes = [P, C.c_int32, C.c_int32, D, D]
         _ref.bhref_blackbody_xyz.argtypes = [D, P]
         _ref.bhref_star_radiance.argtypes = [C.c_int32, C.c_uint64, P, D, P]
         _ref.bhref_polytrope_star.argtypes = [D, D, D, P, P]
@@ -226,6 +228,14 @@ def disc_intensity(scene, threads=0):
     if scene.jet_enabled and scene.jet_model == 1:
         out = out + jet_band_factor(scene.jet_alpha) * c["jet"]
     return out, c
+
+
+def add_glare(xyz, strength, radius_px=3.0):
+    """img::add_glare on an (H, W, 3) image: a new array."""
+    x = np.array(xyz, dtype=np.float64, order="C", copy=True)
+    H, W = x.shape[:2]
+    lib().oracle_add_glare(_ptr(x), W, H, float(strength), float(radius_px))
+    return x
 
 
 def auto_exposure(xyz, key=1.0):

@@ -142,6 +142,17 @@ void bhref_develop(const double* xyz, int64_t npix, double exposure, int tm, dou
     std::memcpy(rgb, out.data(), out.size());
 }
 
+// img::add_glare (image.cpp:47-106) on a row-major w x h XYZ image, in place.
+void bhref_add_glare(double* xyz, int32_t w, int32_t h, double strength, double radius_px) {
+    img::Image im(w, h);
+    const int64_t n = static_cast<int64_t>(w) * h;
+    for (int64_t i = 0; i < n; ++i) im.pixels[i] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+    img::add_glare(im, strength, radius_px);
+    for (int64_t i = 0; i < n; ++i) {
+        xyz[3 * i] = im.pixels[i].x; xyz[3 * i + 1] = im.pixels[i].y; xyz[3 * i + 2] = im.pixels[i].z;
+    }
+}
+
 void bhref_blackbody_xyz(double T, double* o) {
     const spec::XYZ c = spec::blackbody_xyz(T);
     o[0] = c.x; o[1] = c.y; o[2] = c.z;

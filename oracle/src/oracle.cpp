@@ -1576,6 +1576,52 @@ int oracle_generate_catalogue(int32_t count, uint64_t seed, double* x, double* y
     return BHX_OK;
 }
 
+// img::add_glare (image.cpp:47-106): the image plus a narrow (sigma = R) and
+// a wide (3R) separable Gaussian of itself, weights 1 - s, 0.7 s, 0.3 s.
+// Each blur: taps -r..r, r = max(1, ceil(3 sigma)), normalised weights,
+// clamp to the edge; horizontal pass then vertical pass, sums in tap order.
+void oracle_add_glare(double* xyz, int32_t w, int32_t h, double strength, double radius_px) {
+    if (strength <= 0.0 || radius_px <= 0.0) return;
+    const size_t n = static_cast<size_t>(w) * h;
+    auto gauss = [&](double sigma, std::vector<double>& out) {
+        const int r = std::max(1, static_cast<int>(std::ceil(3.0 * sigma)));
+        std::vector<double> k(2 * r + 1);
+        double sum = 0.0;
+        for (int i = -r; i <= r; ++i) {
+            k[i + r] = std::exp(-0.5 * (i * i) / (sigma * sigma));
+            sum += k[i + r];
+        }
+        for (double& v : k) v /= sum;
+        std::vector<double> rows(3 * n);
+        for (int y = 0; y < h; ++y)
+            for (int x = 0; x < w; ++x)
+                for (int c = 0; c < 3; ++c) {
+                    double a = 0.0;
+                    for (int i = -r; i <= r; ++i) {
+                        const int xs = std::min(std::max(x + i, 0), w - 1);
+                        a += k[i + r] * xyz[3 * (static_cast<size_t>(y) * w + xs) + c];
+                    }
+                    rows[3 * (static_cast<size_t>(y) * w + x) + c] = a;
+                }
+        out.assign(3 * n, 0.0);
+        for (int y = 0; y < h; ++y)
+            for (int x = 0; x < w; ++x)
+                for (int c = 0; c < 3; ++c) {
+                    double a = 0.0;
+                    for (int i = -r; i <= r; ++i) {
+                        const int ys = std::min(std::max(y + i, 0), h - 1);
+                        a += k[i + r] * rows[3 * (static_cast<size_t>(ys) * w + x) + c];
+                    }
+                    out[3 * (static_cast<size_t>(y) * w + x) + c] = a;
+                }
+    };
+    std::vector<double> nearb, farb;
+    gauss(radius_px, nearb);
+    gauss(3.0 * radius_px, farb);
+    const double core = 1.0 - strength, wa = 0.7 * strength, wb = 0.3 * strength;
+    for (size_t i = 0; i < 3 * n; ++i) xyz[i] = xyz[i] * core + nearb[i] * wa + farb[i] * wb;
+}
+
 // img::auto_exposure (image.cpp:22-45)
 double oracle_auto_exposure(const double* xyz, int64_t n, double key) {
     std::vector<double> lum;

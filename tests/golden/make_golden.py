@@ -4,7 +4,7 @@ Runs the reference tracer compiled from its own sources (oracle/_ref/
 libbhref.so, see oracle/Makefile) on small scenes and stores its outputs:
 per-ray class, hit state, jet integral, step count, redshift and colour on a
 subset of rays, the SHA-256 of the whole XYZ frame, RenderStats, plus
-colour-table, star-field, display, TOV and jet-normalisation samples.
+colour-table, star-field, display, glare, TOV and jet-normalisation samples.
 Inputs and expected outputs only; nothing of the reference's source.
 
     python tests/golden/make_golden.py
@@ -125,6 +125,17 @@ def main():
         oracle.ref().bhref_develop(xyz.ctypes.data_as(C.c_void_p), len(xyz), e, tm, 16.0,
                                    rgb.ctypes.data_as(C.c_void_p))
         arrays[f"display/rgb{tm}"] = rgb
+    # veiling glare (img::add_glare) on a small image with two bright sources
+    gimg = np.abs(rng.lognormal(0, 2, size=(24, 40, 3))) * 1e-3
+    gimg[5, 7] = 50.0
+    gimg[20, 33] = 10.0
+    arrays["glare/xyz"] = gimg
+    meta["glare"] = []
+    for k, (s, rad) in enumerate(((0.3, 3.0), (0.1, 1.5), (0.5, 0.7))):
+        out = np.array(gimg, copy=True)
+        oracle.ref().bhref_add_glare(out.ctypes.data_as(C.c_void_p), 40, 24, s, rad)
+        arrays[f"glare/out{k}"] = out
+        meta["glare"].append({"strength": s, "radius_px": rad})
     # TOV: Gamma = 2, K = 100 (G = c = Msun = 1) polytrope, K in SI for P = K rho^2
     L = G * M_SUN / C_LIGHT ** 2
     K_si = 100.0 * L ** 3 * C_LIGHT ** 2 / M_SUN

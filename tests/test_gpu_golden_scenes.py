@@ -74,3 +74,31 @@ def test_colour_table_matches_reference(tracer):
     rel = np.abs(out - ref) / np.maximum(np.abs(ref), 1e-300)
     print(f"[colour table] max rel {rel.max():.3e}", flush=True)
     assert rel.max() < 1e-12
+
+
+@pytest.mark.parametrize("k", range(3))
+def test_glare_matches_reference(tracer, k):
+    """bhx_add_glare_dev against the reference's img::add_glare outputs: bit
+    for bit (no FMA, host-computed taps, the reference's summation order)."""
+    import torch
+    g = META["glare"][k]
+    x = torch.from_numpy(np.ascontiguousarray(ARR["glare/xyz"])).to(f"cuda:{tracer.device}")
+    tracer.add_glare(x, g["strength"], g["radius_px"])
+    out = x.cpu().numpy()
+    assert np.array_equal(out, ARR[f"glare/out{k}"]), (g, np.abs(out - ARR[f"glare/out{k}"]).max())
+
+
+def test_glare_frame_matches_oracle(tracer):
+    """A rendered 96 x 64 frame through the GPU glare and the oracle's."""
+    import torch
+    from blackhole_amd import scene as S
+    sc = S.starry_frame(96, 64)
+    xyz, _ = tracer.render(sc)
+    x = torch.from_numpy(np.ascontiguousarray(xyz)).to(f"cuda:{tracer.device}")
+    for strength, radius in ((0.2, 3.0), (0.05, 9.5)):
+        y = x.clone()
+        tracer.add_glare(y, strength, radius)
+        assert np.array_equal(y.cpu().numpy(), oracle.add_glare(xyz, strength, radius)), (strength, radius)
+    y = x.clone()
+    tracer.add_glare(y, 0.0, 3.0)                      # strength 0: untouched (image.cpp:48)
+    assert torch.equal(y, x)

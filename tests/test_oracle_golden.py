@@ -75,3 +75,12 @@ def test_display_transform():
     assert e == META["display"]["exposure"]
     for tm in range(5):
         assert np.array_equal(oracle.develop(xyz, e, tm, 16.0), ARR[f"display/rgb{tm}"]), tm
+
+
+@pytest.mark.parametrize("k", range(3))
+def test_glare(k):
+    """img::add_glare (image.cpp:47-106): two separable Gaussians, clamp to edge."""
+    g = META["glare"][k]
+    out = oracle.add_glare(ARR["glare/xyz"], g["strength"], g["radius_px"])
+    assert np.array_equal(out, ARR[f"glare/out{k}"]), g
+    assert not np.array_equal(out, ARR["glare/xyz"])
