@@ -144,10 +144,9 @@ struct Rhs {
 // with H = g^{ab} p_a p_b.  Algebraically identical to the component-wise
 // partials of the reference (checked against the oracle's RHS to rounding),
 // about 40% fewer operations, and a single reciprocal of Sigma Delta sin^2.
-__host__ __device__ __forceinline__ void kerr_rhs(const RayK& K, double r, double th, double pr, double pth,
-                                         Rhs& o) {
-    double sn, cs;
-    sincos_near(th, &sn, &cs);
+// The same with sin and cos of theta given (sn, cs).
+__host__ __device__ __forceinline__ void kerr_rhs_sc(const RayK& K, double r, double sn, double cs, double pr,
+                                                     double pth, Rhs& o) {
     if (fabs(sn) < 1e-9) sn = (sn < 0.0) ? -1e-9 : 1e-9;  // kerr.h:34,51
     const double sn2 = sn * sn;
     const double r2 = r * r;
@@ -181,6 +180,13 @@ __host__ __device__ __forceinline__ void kerr_rhs(const RayK& K, double r, doubl
     o.iD = iD;
     o.D = D;
     o.rpa = rpa;
+}
+
+__host__ __device__ __forceinline__ void kerr_rhs(const RayK& K, double r, double th, double pr, double pth,
+                                         Rhs& o) {
+    double sn, cs;
+    sincos_near(th, &sn, &cs);
+    kerr_rhs_sc(K, r, sn, cs, pr, pth, o);
 }
 
 // splitmix64 (the reference's Rng, src/scene.h:27-44), counter form: draw n
