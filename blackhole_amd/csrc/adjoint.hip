@@ -2,9 +2,12 @@
 // kernel and the discrete-adjoint kernel over the per-ray pieces of
 // adjoint_dev.h, behind the bhx_adjoint_* entry points.
 //
-//   k_adj_forward   one ray per lane; a lane whose ray ends takes the next ray
-//                   index from a global counter in the same iteration, so every
-//                   iteration is one DP5 attempt on every lane.  Writes the
+//   k_adj_forward   one ray per lane; a lane whose ray ends takes the next queue
+//                   slot from a global counter in the same iteration, so every
+//                   iteration is one DP5 attempt on every lane.  Queue slots map
+//                   to rays in the tracer's queue order (8 x 8-pixel tiles,
+//                   centre first, the spin axis' image first with the conical
+//                   jet), so a wave holds neighbouring rays of similar cost.  Writes the
 //                   class, hit state, accepted-step count and intensity
 //                   (disc plus jet) of each sample, and a checkpoint of the
 //                   integrator state every K accepted steps (HBM slab of
@@ -34,6 +37,7 @@ struct AdjArgs {
     int64_t n_rays;
     // forward
     uint32_t* counter;
+    const int32_t* qmap;      // queue slot -> ray index
     double* ck;               // n_rays * cap * kCk
     uint8_t* cls;
     double* y;                // 8 x n_rays SoA (layout of bhx_rays.y)
@@ -68,7 +72,7 @@ __global__ void __launch_bounds__(256) k_adj_forward(const AdjConst* __restrict_
             if ((int64_t)r >= P.n_rays) {
                 done = true;
             } else {
-                ray = r;
+                ray = P.qmap[r];
                 fwd_init(F, G, ray, s, P.ck + ray * slab);
                 live = true;
             }
@@ -137,7 +141,7 @@ struct bhx_adjoint {
     int K = 0;
     int64_t n_rays = 0;
     AdjConst host{};
-    DevBuf cst, ck, cls, y, steps, I, key, key_sorted, iota, order, sort_tmp, seg, partial, counters;
+    DevBuf cst, ck, cls, y, steps, I, key, key_sorted, iota, order, sort_tmp, seg, partial, counters, qmap;
     int grid_f = 0, grid_b = 0;
     double fwd_ms = 0.0, bwd_ms = 0.0;
     bool fwd_pending = false;
@@ -216,6 +220,26 @@ int bhx_adjoint_create(bhx_context* ctx, const bhx_scene* s, int32_t K, bhx_adjo
         return bail(bhx_fail(ctx, BHX_E_NOMEM, "adjoint buffers"));
     for (auto& e : A->ev)
         if (hipEventCreate(&e) != hipSuccess) return bail(bhx_fail(ctx, BHX_E_HIP, "hipEventCreate"));
+    // the forward's queue order: the tracer's tile order as a slot -> ray map
+    {
+        std::vector<int32_t> order, start;
+        host::tile_queue(s->width, s->height, S, 0, order, start, host::axis_band(*s), s->cam_roll);
+        std::vector<int32_t> map(static_cast<size_t>(n));
+        const int tiles_x = (s->width + 7) / 8, S2 = S * S;
+        for (size_t t = 0; t < order.size(); ++t) {
+            const int tx = order[t] % tiles_x, ty = order[t] / tiles_x;
+            const int x0 = 8 * tx, y0 = 8 * ty;
+            const int tw = std::min(8, s->width - x0), th = std::min(8, s->height - y0);
+            for (int i = 0; i < tw * th * S2; ++i) {
+                const int pin = i / S2, k = i - pin * S2;
+                const int px = x0 + pin % tw, py = y0 + pin / tw;
+                map[static_cast<size_t>(start[t]) + i] = (py * s->width + px) * S2 + k;
+            }
+        }
+        if (A->qmap.ensure(sizeof(int32_t) * n) != hipSuccess) return bail(bhx_fail(ctx, BHX_E_NOMEM, "queue map"));
+        if (hipMemcpy(A->qmap.p, map.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(bhx_fail(ctx, BHX_E_HIP, "queue map upload"));
+    }
     // ray ids 0..n-1 (sort values) and the radix sort's scratch
     hipLaunchKernelGGL(k_iota, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, 0,
                        static_cast<int32_t*>(A->iota.p), n);
@@ -239,7 +263,7 @@ int bhx_adjoint_destroy(bhx_adjoint* A) {
     (void)hipSetDevice(A->ctx->device);
     (void)hipDeviceSynchronize();
     for (DevBuf* d : {&A->cst, &A->ck, &A->cls, &A->y, &A->steps, &A->I, &A->key, &A->key_sorted, &A->iota,
-                      &A->order, &A->sort_tmp, &A->seg, &A->partial, &A->counters})
+                      &A->order, &A->sort_tmp, &A->seg, &A->partial, &A->counters, &A->qmap})
         d->release();
     for (auto& e : A->ev)
         if (e) (void)hipEventDestroy(e);
@@ -262,6 +286,7 @@ int bhx_adjoint_forward(bhx_adjoint* A, double* I, void* stream) {
     AdjArgs P{};
     P.n_rays = A->n_rays;
     P.counter = static_cast<uint32_t*>(A->counters.p);
+    P.qmap = static_cast<const int32_t*>(A->qmap.p);
     P.ck = static_cast<double*>(A->ck.p);
     P.cls = static_cast<uint8_t*>(A->cls.p);
     P.y = static_cast<double*>(A->y.p);
