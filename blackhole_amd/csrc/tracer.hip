@@ -37,6 +37,9 @@ namespace {
 using namespace dp5;
 
 constexpr int kTile = 8;
+// per-lane LDS slot of a waiting conical-jet sample: dense-output coefficients
+// of r, theta, p_r (5 each), s_jet, h
+constexpr int kPend = 17;
 
 // Four-point Gauss-Legendre nodes and weights on [0, 1] (render.cpp:47-50).
 __constant__ const double kGLx[4] = {0.06943184420297371, 0.33000947820757187, 0.66999052179242813,
@@ -109,7 +112,7 @@ __global__ void __launch_bounds__(64) k_raygen(const RaygenArgs P) {
 #define SK(f) (kBatch ? sk[sid].f : P.f)
 #define EV (kBatch ? sk[sid].ev : *P.ev)
 template <bool kBatch>
-__device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
+__device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk, double* pend,
                                            unsigned long long* blk) {
     // ---- per-scene constants -------------------------------------------------
     double la = P.a, la2 = P.a2, lrtol = P.rtol, latol = P.atol;
@@ -123,6 +126,11 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
     double h = 0.0, pw = 0.0, c_prev = 0.0, q0raw = 0.0, iq0 = 0.0;
     double mx_norm = 0.0, mx_carter = 0.0, jet_acc = 0.0;
     int nsteps = 0, natt = 0, tries = 0;
+    // conical jet: a sample of the last step waits (jet_pend) in the lane's LDS
+    // slot my_pend; a ray that ended writes its jet integral at the next loop
+    // head (jet_out), after that sample
+    bool jet_pend = false, jet_out = false;
+    double* my_pend = pend + threadIdx.x * kPend;
 #pragma unroll
     for (int c = 0; c < 6; ++c) { y[c] = 0.0; k1[c] = 0.0; }
 
@@ -152,6 +160,92 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
     const int64_t N = P.n_rays;
 
     for (;;) {
+        // ---- conical jet (model 1): kConeSub x 4-point Gauss-Legendre -------
+        // on the dense output over [0, s_jet] of each lane's last step.  A
+        // sampled step costs ~10 plain steps, so the wave shares the nodes:
+        // here at the loop head every lane is active, each pass serves two
+        // sampling lanes (owners), 32 lanes per owner evaluate one node each
+        // from the owner's LDS slot, and a butterfly sums each half - the
+        // balanced tree over the nodes in order.  (A wave that is not whole
+        // here lets each owner sum its own nodes along the same tree.)
+        {
+            const unsigned long long owners = __ballot(jet_pend);
+            if (owners != 0ull) {
+                const bool coop = __ballot(true) == ~0ull;
+                unsigned long long rem = owners;
+                double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0, l4 = 0.0, jsum = 0.0;
+                const int n_pass = coop ? (__popcll(owners) + 1) / 2 : kConeSub * 4;
+#pragma unroll 1
+                for (int pass = 0; pass < n_pass; ++pass) {
+                    int owner = lane, j = pass, A = -1, B = -1;
+                    if (coop) {
+                        A = __builtin_ctzll(rem);
+                        rem &= rem - 1;
+                        if (rem) {
+                            B = __builtin_ctzll(rem);
+                            rem &= rem - 1;
+                        }
+                        owner = (lane >= 32 && B >= 0) ? B : A;
+                        j = lane & 31;
+                    }
+                    const double* op = pend + (threadIdx.x - lane + owner) * kPend;
+                    const double opt = __shfl(pt, owner), opph = __shfl(pph, owner);
+                    double ola = la, ola2 = la2;
+                    int osid = 0;
+                    if constexpr (kBatch) {
+                        osid = __shfl(sid, owner);
+                        ola = __shfl(la, owner);
+                        ola2 = __shfl(la2, owner);
+                    }
+                    const EventParams& JO = kBatch ? sk[osid].ev : *P.ev;
+                    // node j: sub-interval j / 4, Gauss point j % 4
+                    const int qd = j & 3;
+                    const double sv = ((j >> 2) + kGLx[qd]) * (1.0 / kConeSub) * op[15];
+                    const double rs = dense_comp(op[0], op[1], op[2], op[3], op[4], sv);
+                    const double ths = dense_comp(op[5], op[6], op[7], op[8], op[9], sv);
+                    double sn, cs;
+                    sincos_near(ths, &sn, &cs);
+                    double v = 0.0;
+                    const double sh = cone_shape(JO, rs, cs);
+                    if (sh > 0.0) {
+                        const double prs = dense_comp(op[10], op[11], op[12], op[13], op[14], sv);
+                        const double b = cone_boost(JO, ola, ola2, rs, sn, cs, opt, prs, opph);
+                        if (b > 0.0) v = kGLw[qd] * sh * b;
+                    }
+                    if (coop) {
+#pragma unroll
+                        for (int off = 1; off < 32; off <<= 1) v = v + __shfl_xor(v, off);
+                        const double sA = __shfl(v, 0), sB = __shfl(v, 32);
+                        if (lane == A) jsum = sA;
+                        if (lane == B) jsum = sB;
+                    } else {
+                        // node j closes the subtrees of its trailing one bits
+                        const int t = __builtin_ctz(~j);
+                        double c = v;
+                        if (t > 0) c = l0 + c;
+                        if (t > 1) c = l1 + c;
+                        if (t > 2) c = l2 + c;
+                        if (t > 3) c = l3 + c;
+                        if (t > 4) c = l4 + c;
+                        if (t == 0) l0 = c;
+                        else if (t == 1) l1 = c;
+                        else if (t == 2) l2 = c;
+                        else if (t == 3) l3 = c;
+                        else if (t == 4) l4 = c;
+                        else jsum = c;
+                    }
+                }
+                if (jet_pend) {
+                    jet_acc += EV.j_nu0 * jsum * my_pend[16] * my_pend[15] * (1.0 / kConeSub);
+                    jet_pend = false;
+                }
+            }
+        }
+        if (jet_out) {   // the ray ended on the last step
+            if (P.jet) P.jet[P.q2ray[q]] = jet_acc;
+            jet_out = false;
+        }
+
         // ---- refill idle lanes from the wave's chunk -----------------------
         unsigned long long idle = __ballot(!live);
         const int n_idle = __popcll(idle);
@@ -396,29 +490,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
                     }
                 }
                 if (term < 0) s_jet = 1.0;   // nothing stopped the ray: the whole step counts
-                if (near_jet && s_jet > 0.0 && EVS(model) == 1) {
-                    // ---- conical jet: kConeSub x 4-point Gauss-Legendre
-                    const EventParams& J = EV;
-                    double sum = 0.0;
-#pragma unroll 1
-                    for (int ks = 0; ks < kConeSub; ++ks) {
-#pragma unroll 1
-                        for (int qd = 0; qd < 4; ++qd) {
-                            const double sv = (ks + kGLx[qd]) * (1.0 / kConeSub) * s_jet;
-                            const double rs = dense_comp(c1[1], c2[1], c3[1], c4[1], c5[1], sv);
-                            const double ths = dense_comp(c1[2], c2[2], c3[2], c4[2], c5[2], sv);
-                            double sn, cs;
-                            sincos_near(ths, &sn, &cs);
-                            const double sh = cone_shape(J, rs, cs);
-                            if (sh <= 0.0) continue;
-                            const double prs = dense_comp(c1[4], c2[4], c3[4], c4[4], c5[4], sv);
-                            const double b = cone_boost(J, la, la2, rs, sn, cs, pt, prs, pph);
-                            if (b <= 0.0) continue;
-                            sum += kGLw[qd] * sh * b;
-                        }
-                    }
-                    jet_acc += J.j_nu0 * sum * h * s_jet * (1.0 / kConeSub);
-                } else if (near_jet && s_jet > 0.0) {
+                if (near_jet && s_jet > 0.0 && EVS(model) != 1) {
                     // ---- jet emission: 4-point Gauss-Legendre (render.cpp:107-122)
                     const EventParams& J = EV;
                     double sum = 0.0;
@@ -439,6 +511,26 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
                     }
                     jet_acc += J.j_nu0 * sum * h * s_jet;
                 }
+            }
+            if (near_jet && s_jet > 0.0 && EVS(model) == 1) {
+                // ---- conical jet (model 1): the sample is taken at the next loop
+                // head, where the whole wave is converged (see there); its dense
+                // output (r, theta, p_r), s_jet and h wait in the lane's LDS slot
+                const int cc[3] = {1, 2, 4};
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    const int c = cc[m];
+                    const double d2 = yn[c] - y[c];
+                    const double d3 = h * k1[c] - d2;
+                    my_pend[5 * m] = y[c];
+                    my_pend[5 * m + 1] = d2;
+                    my_pend[5 * m + 2] = d3;
+                    my_pend[5 * m + 3] = d2 - h * g7.d[c] - d3;
+                    my_pend[5 * m + 4] = c5[c];
+                }
+                my_pend[15] = s_jet;
+                my_pend[16] = h;
+                jet_pend = true;
             }
             if (term < 0 && yn[1] > SK(r_escape)) {
                 // ---- escape (render.cpp:247-251), after the jet sample ----------
@@ -477,7 +569,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
                 P.y[3 * M + ray] = yo[3]; P.y[4 * M + ray] = pt; P.y[5 * M + ray] = yo[4];
                 P.y[6 * M + ray] = yo[5]; P.y[7 * M + ray] = pph;
             }
-            if (P.jet) P.jet[ray] = jet_acc;
+            jet_out = true;   // the jet integral is written at the next loop head
             if (P.steps) P.steps[ray] = nsteps;
             if (P.attempts) P.attempts[ray] = natt;
             if (P.norm_err) P.norm_err[ray] = mx_norm;
@@ -509,9 +601,10 @@ __device__ __forceinline__ void flush_totals(const TraceArgs& P, unsigned long l
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 k_trace(const TraceArgs P) {
     __shared__ unsigned long long blk[4];  // steps, attempts, max norm bits, max Carter bits
+    extern __shared__ double pend_raw[];   // kPend doubles per lane: waiting jet samples
     if (threadIdx.x < 4) blk[threadIdx.x] = 0ull;
     __syncthreads();
-    trace_body<false>(P, nullptr, blk);
+    trace_body<false>(P, nullptr, pend_raw, blk);
     flush_totals(P, blk);
 }
 
@@ -528,7 +621,7 @@ k_trace_batch(const TraceArgs P) {
         for (int i = threadIdx.x; i < nd; i += blockDim.x) sk_raw[i] = src[i];
     }
     __syncthreads();
-    trace_body<true>(P, sk, blk);
+    trace_body<true>(P, sk, sk_raw + P.n_scenes * (sizeof(SceneK) / sizeof(double)), blk);
     flush_totals(P, blk);
 }
 
@@ -540,24 +633,27 @@ hipError_t launch_raygen(const RaygenArgs& args, hipStream_t stream) {
 }
 
 // Host launchers: one persistent grid.
+// Dynamic LDS: [scene table (batched only)] [kPend doubles per lane].
+static size_t pend_bytes(int block) { return sizeof(double) * kPend * (size_t)block; }
+
 hipError_t launch_trace(const TraceArgs& args, int grid, int block, hipStream_t stream) {
-    hipLaunchKernelGGL(k_trace, dim3(grid), dim3(block), 0, stream, args);
+    hipLaunchKernelGGL(k_trace, dim3(grid), dim3(block), pend_bytes(block), stream, args);
     return hipGetLastError();
 }
 
 hipError_t launch_trace_batch(const TraceArgs& args, int grid, int block, hipStream_t stream) {
-    const size_t lds = sizeof(SceneK) * (size_t)args.n_scenes;
+    const size_t lds = sizeof(SceneK) * (size_t)args.n_scenes + pend_bytes(block);
     hipLaunchKernelGGL(k_trace_batch, dim3(grid), dim3(block), lds, stream, args);
     return hipGetLastError();
 }
 
 hipError_t trace_occupancy(int block, int* blocks_per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_trace, block, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_trace, block, pend_bytes(block));
 }
 
 hipError_t trace_batch_occupancy(int block, int n_scenes, int* blocks_per_cu) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_trace_batch, block,
-                                                        sizeof(SceneK) * (size_t)n_scenes);
+                                                        sizeof(SceneK) * (size_t)n_scenes + pend_bytes(block));
 }
 
 }  // namespace bhx
