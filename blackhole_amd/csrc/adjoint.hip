@@ -14,7 +14,8 @@
 //                   cap = max_steps / K + 1 records of 8 doubles per ray).
 //   k_adj_backward  one lit ray (I > 0) per lane, sorted by step count so the
 //                   lanes of a wave replay and sweep segments of equal length
-//                   together; each lane's (K + 1)-step segment buffer is a strided
+//                   together (a wave's few jet-lit lanes share their jet
+//                   samples' node evaluations with the other lanes); each lane's (K + 1)-step segment buffer is a strided
 //                   HBM scratch.  Per-block partial gradients, summed on the
 //                   host in a fixed order.
 #include "adjoint_frame.h"
@@ -59,14 +60,63 @@ struct AdjConst {
     RaygenArgs G;
 };
 
-__global__ void __launch_bounds__(256) k_adj_forward(const AdjConst* __restrict__ C, const AdjArgs P) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+k_adj_forward(const AdjConst* __restrict__ C, const AdjArgs P) {
+    // A step's conical-jet sample (32 nodes, ~10 steps of work) is not taken in
+    // the step: its dense output waits in the lane's LDS slot, and at the loop
+    // head, where every lane of the wave is active, each pass serves two
+    // waiting lanes with one node per lane; the owner then sums its nodes in
+    // order, as jet_step does.  A ray that ended writes its intensity after
+    // its last sample.
+    __shared__ double pend[256 * 17];
+    __shared__ double nodes[4][64];
     const Frame& F = C->F;
     const RaygenArgs& G = C->G;
+    const int lane = __lane_id();
+    double* my_pend = pend + threadIdx.x * 17;
+    double* wave_nodes = nodes[threadIdx.x / 64];
     Ray s;
-    bool live = false, done = false;
+    s.jet_wait = 0;
+    s.pt = s.pph = s.jet = 0.0;
+    bool live = false, done = false, out_wait = false;
+    double I_disc = 0.0;
     int64_t ray = 0;
     const int64_t slab = static_cast<int64_t>(F.cap) * kCk;
     for (;;) {
+        unsigned long long owners = __ballot(s.jet_wait != 0);
+        while (owners != 0ull) {
+            const int A = __builtin_ctzll(owners);
+            owners &= owners - 1;
+            int B = -1;
+            if (owners != 0ull) {
+                B = __builtin_ctzll(owners);
+                owners &= owners - 1;
+            }
+            const int owner = (lane >= 32 && B >= 0) ? B : A;
+            const double* op = pend + (threadIdx.x - lane + owner) * 17;
+            const double opt = __shfl(s.pt, owner), opph = __shfl(s.pph, owner);
+            wave_nodes[lane] = jet_value_node(F, op, op[15], opt, opph, lane & 31);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (lane == A || lane == B) {
+                const double* v = wave_nodes + ((lane == A) ? 0 : 32);
+                double sum = 0.0;
+                for (int j = 0; j < 4 * kConeSub; ++j) sum += gl_w(j & 3) * v[j];
+                s.jet += F.j0 * sum * my_pend[16] * my_pend[15] / kConeSub;
+                s.jet_wait = 0;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        if (out_wait) {   // the ray ended on the last step
+            const double I = I_disc + F.kappa * s.jet;
+            P.I[ray] = I;
+            P.key[ray] = (I > 0.0) ? max(1, min(s.nsteps, 32767)) : 0;   // 15-bit key, >= 1 for a lit ray
+            if (I > 0.0) atomicAdd(P.n_lit, 1u);
+            out_wait = false;
+        }
         if (!live && !done) {
             const uint32_t r = atomicAdd(P.counter, 1u);
             if ((int64_t)r >= P.n_rays) {
@@ -80,17 +130,15 @@ __global__ void __launch_bounds__(256) k_adj_forward(const AdjConst* __restrict_
         if (__ballot(live) == 0ull) break;
         if (!live) continue;
         double yo[6], se = 0.0;
-        const int term = fwd_step(F, s, P.ck + ray * slab, yo, &se);
+        const int term = fwd_step(F, s, P.ck + ray * slab, yo, &se, my_pend);
         if (term >= 0) {
             const int64_t N = P.n_rays;
             P.cls[ray] = (uint8_t)term;
             P.y[ray] = yo[0]; P.y[N + ray] = yo[1]; P.y[2 * N + ray] = yo[2]; P.y[3 * N + ray] = yo[3];
             P.y[4 * N + ray] = s.pt; P.y[5 * N + ray] = yo[4]; P.y[6 * N + ray] = yo[5]; P.y[7 * N + ray] = s.pph;
             P.steps[ray] = s.nsteps;
-            const double I = ((term == BHX_DISC) ? disc_intensity(F, yo[1], s.pt, s.pph) : 0.0) + F.kappa * s.jet;
-            P.I[ray] = I;
-            P.key[ray] = (I > 0.0) ? max(1, min(s.nsteps, 32767)) : 0;   // 15-bit key, >= 1 for a lit ray
-            if (I > 0.0) atomicAdd(P.n_lit, 1u);
+            I_disc = (term == BHX_DISC) ? disc_intensity(F, yo[1], s.pt, s.pph) : 0.0;
+            out_wait = true;
             live = false;
         }
     }
@@ -103,6 +151,7 @@ __global__ void k_iota(int32_t* p, int64_t n) {
 
 __global__ void __launch_bounds__(256) k_adj_backward(const AdjConst* __restrict__ C, const AdjArgs P) {
     __shared__ double part[4][4];
+    __shared__ double jet_lds[4][kJetLds];   // per wave: the jet nodes shared across its lanes
     const Frame& F = C->F;
     const RaygenArgs& G = C->G;
     const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -115,7 +164,8 @@ __global__ void __launch_bounds__(256) k_adj_backward(const AdjConst* __restrict
         const int64_t ray = P.order[i];
         const double w = P.w[ray];
         if (w == 0.0) continue;
-        mism += bwd_ray(F, G, ray, P.steps[ray], P.cls[ray], w, P.ck + ray * slab, P.seg + lane, P.seg_stride, g);
+        mism += bwd_ray(F, G, ray, P.steps[ray], P.cls[ray], w, P.ck + ray * slab, P.seg + lane, P.seg_stride, g,
+                        F.has_jet ? jet_lds[threadIdx.x / 64] : nullptr);
     }
     // block sums in a fixed order
 #pragma unroll

@@ -61,6 +61,7 @@ struct Frame {
 struct Ray {
     double y[6], k1[6], pt, pph, h, pw, c_prev, jet;
     int nsteps, tries;
+    int jet_wait;   // a jet sample of the last step waits in the caller's slot (see fwd_step)
 };
 
 __host__ __device__ inline void write_ck(double* ck, const Ray& s) {
@@ -307,20 +308,27 @@ __host__ __device__ inline double gl_w(int q) { return (q == 0 || q == 3) ? 0.17
 // dense output cf (components r, theta, p_r, p_theta): kConeSub
 // sub-intervals of 4-point Gauss-Legendre, j0 h s_end sum w_q F / kConeSub
 // (the tracer's sampling of the conical jet).
+// The integrand at node j (sub-interval j / 4, Gauss point j % 4) of a step's
+// jet sample over [0, s_end] of the dense output c15 (r, theta, p_r: 5
+// coefficients each).
+__host__ __device__ inline double jet_value_node(const Frame& F, const double* c15, double s_end, double pt,
+                                                 double pph, int j) {
+    const double u = ((j >> 2) + gl_x(j & 3)) / kConeSub;
+    const double sv = u * s_end;
+    const double r = dense_comp(c15[0], c15[1], c15[2], c15[3], c15[4], sv);
+    const double th = dense_comp(c15[5], c15[6], c15[7], c15[8], c15[9], sv);
+    const double pr = dense_comp(c15[10], c15[11], c15[12], c15[13], c15[14], sv);
+    return cone_F<double>(F, r, th, pr, F.a, pt, pph);
+}
+
 __host__ __device__ inline double jet_step(const Frame& F, const double cf[4][5], double h, double s_end,
                                            double pt, double pph) {
+    double c15[15];
+    for (int c = 0; c < 3; ++c)
+        for (int m = 0; m < 5; ++m) c15[5 * c + m] = cf[c][m];
     double sum = 0.0;
 #pragma unroll 1
-    for (int k = 0; k < kConeSub; ++k)
-#pragma unroll 1
-    for (int q = 0; q < 4; ++q) {
-        const double u = (k + gl_x(q)) / kConeSub;
-        const double sv = u * s_end;
-        const double r = dense_comp(cf[0][0], cf[0][1], cf[0][2], cf[0][3], cf[0][4], sv);
-        const double th = dense_comp(cf[1][0], cf[1][1], cf[1][2], cf[1][3], cf[1][4], sv);
-        const double pr = dense_comp(cf[2][0], cf[2][1], cf[2][2], cf[2][3], cf[2][4], sv);
-        sum += gl_w(q) * cone_F<double>(F, r, th, pr, F.a, pt, pph);
-    }
+    for (int j = 0; j < 4 * kConeSub; ++j) sum += gl_w(j & 3) * jet_value_node(F, c15, s_end, pt, pph, j);
     return F.j0 * sum * h * s_end / kConeSub;
 }
 
@@ -508,39 +516,150 @@ __host__ __device__ inline void dense_coef_vjp(double h, const double cbar[5][4]
     }
 }
 
+// Node j (sub-interval j / 4, Gauss point j % 4) of a step's jet sample over
+// [0, s_end] of the dense output cf3 (components r, theta, p_r): the
+// integrand and its partials in (r, theta, p_r, a, p_t, p_phi), and the
+// s-derivatives of (r, theta, p_r) there.
+struct JetNode {
+    double f[7];    // value, d/d(r, theta, p_r, a, p_t, p_phi)
+    double yp[3];   // d(r, theta, p_r)/ds
+};
+__host__ __device__ inline void jet_node(const Frame& F, const double cf3[3][5], double s_end, double pt, double pph,
+                                         int j, JetNode& o) {
+    using D6 = Dual<6>;
+    const int k = j >> 2, q = j & 3;
+    const double u = (k + gl_x(q)) / kConeSub;
+    const double sv = u * s_end;
+    double Y[3];
+    for (int c = 0; c < 3; ++c) dense_vd(cf3[c][0], cf3[c][1], cf3[c][2], cf3[c][3], cf3[c][4], sv, Y[c], o.yp[c]);
+    const D6 f = cone_F<D6>(F, D6::var(Y[0], 0), D6::var(Y[1], 1), D6::var(Y[2], 2), D6::var(F.a, 3),
+                            D6::var(pt, 4), D6::var(pph, 5));
+    o.f[0] = f.v;
+    for (int m = 0; m < 6; ++m) o.f[m + 1] = f.d[m];
+}
+
+// Adds node j's share of the seeds of wk J_step (coef = wk j0 h): the node
+// seeds on the dense-output coefficients (cbar), the constants' (cb), and its
+// terms of J_step's node sum and of d(wk J_step)/ds_end.
+__host__ __device__ inline void jet_accum(double coef, double s_end, int j, const JetNode& n, double cbar[5][4],
+                                          double cb[3], double* sum, double* dJds) {
+    if (n.f[0] == 0.0) return;
+    const int k = j >> 2, q = j & 3;
+    const double u = (k + gl_x(q)) / kConeSub;
+    const double sv = u * s_end;
+    *sum += gl_w(q) * n.f[0];
+    const double wq = coef * s_end * gl_w(q) / kConeSub;
+    const double Yb[4] = {wq * n.f[1], wq * n.f[2], wq * n.f[3], 0.0};
+    dense_seed(sv, Yb, cbar);
+    cb[0] += wq * n.f[4];
+    cb[1] += wq * n.f[5];
+    cb[2] += wq * n.f[6];
+    *dJds += gl_w(q) / kConeSub * (n.f[0] + s_end * u * (n.f[1] * n.yp[0] + n.f[2] * n.yp[1] + n.f[3] * n.yp[2]));
+}
+
 // Adjoint seeds of wk * J_step, J_step = j0 h s_end sum_q w_q F(Y(x_q s_end)),
 // on the step's dense output: node seeds into cbar, constants (a, p_t, p_phi)
 // into cb.  Returns wk dJ_step/ds_end (for an event-truncated step) and
-// J_step in *Js.
+// J_step in *Js.  The nodes in order, one lane.
 __host__ __device__ inline double jet_seeds(const Frame& F, const double cf[4][5], double h, double s_end,
                                             double wk, double pt, double pph, double cbar[5][4], double cb[3],
                                             double* Js) {
-    using D6 = Dual<6>;
     const double coef = wk * F.j0 * h;
+    const double cf3[3][5] = {{cf[0][0], cf[0][1], cf[0][2], cf[0][3], cf[0][4]},
+                              {cf[1][0], cf[1][1], cf[1][2], cf[1][3], cf[1][4]},
+                              {cf[2][0], cf[2][1], cf[2][2], cf[2][3], cf[2][4]}};
     double dJds = 0.0, sum = 0.0;
 #pragma unroll 1
-    for (int k = 0; k < kConeSub; ++k)
-#pragma unroll 1
-    for (int q = 0; q < 4; ++q) {
-        const double u = (k + gl_x(q)) / kConeSub;
-        const double sv = u * s_end;
-        double Y[4], Yp[4];
-        for (int c = 0; c < 4; ++c) dense_vd(cf[c][0], cf[c][1], cf[c][2], cf[c][3], cf[c][4], sv, Y[c], Yp[c]);
-        const D6 f = cone_F<D6>(F, D6::var(Y[0], 0), D6::var(Y[1], 1), D6::var(Y[2], 2), D6::var(F.a, 3),
-                                D6::var(pt, 4), D6::var(pph, 5));
-        if (f.v == 0.0) continue;
-        sum += gl_w(q) * f.v;
-        const double wq = coef * s_end * gl_w(q) / kConeSub;
-        const double Yb[4] = {wq * f.d[0], wq * f.d[1], wq * f.d[2], 0.0};
-        dense_seed(sv, Yb, cbar);
-        cb[0] += wq * f.d[3];
-        cb[1] += wq * f.d[4];
-        cb[2] += wq * f.d[5];
-        dJds += gl_w(q) / kConeSub * (f.v + s_end * u * (f.d[0] * Yp[0] + f.d[1] * Yp[1] + f.d[2] * Yp[2]));
+    for (int j = 0; j < 4 * kConeSub; ++j) {
+        JetNode n;
+        jet_node(F, cf3, s_end, pt, pph, j, n);
+        jet_accum(coef, s_end, j, n, cbar, cb, &sum, &dJds);
     }
     *Js = F.j0 * sum * h * s_end / kConeSub;
     return coef * dJds;
 }
+
+// LDS doubles per wave of jet_seeds_wave: two owners' dense output and
+// constants, and their nodes' partials.
+constexpr int kJetLds = 2 * 18 + 2 * 4 * kConeSub * 10;
+constexpr int kJetShareMax = 8;   // share the nodes when at most this many lanes sample
+#if defined(__HIP_DEVICE_COMPILE__)
+// jet_seeds with the wave's active lanes sharing the node evaluations: each
+// round takes up to two lanes that need a sample (owners), publishes their
+// dense output in LDS, spreads their 4 kConeSub nodes over the active lanes,
+// and each owner then accumulates its nodes in order - the same arithmetic
+// as jet_seeds, so the seeds are the same bits.  All active lanes must call
+// it; `need` marks the owners, whose other arguments are used.  lds: this
+// wave's kJetLds doubles.
+__device__ inline double jet_seeds_wave(const Frame& F, const double cf[4][5], double h, double s_end, double wk,
+                                        double pt, double pph, double cbar[5][4], double cb[3], double* Js,
+                                        bool need, double* lds) {
+    const unsigned long long act = __ballot(true);
+    unsigned long long rem = __ballot(need);
+    const int n_act = __popcll(act);
+    const int lane = __lane_id();
+    const int rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(act >> 32),
+                                               __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(act), 0u));
+    const int per = n_act >= 64 ? 2 : 1;   // owners per round
+    double* own_lds = lds;                  // [2][18]: cf3 (15), s_end, pt, pph
+    double* node_lds = lds + 2 * 18;        // [2][4 kConeSub][10]: f (7), yp (3)
+    double Os = 0.0;
+    while (rem != 0ull) {
+        int own[2] = {-1, -1};
+        own[0] = __builtin_ctzll(rem);
+        rem &= rem - 1;
+        if (per == 2 && rem != 0ull) {
+            own[1] = __builtin_ctzll(rem);
+            rem &= rem - 1;
+        }
+        const int n_own = own[1] >= 0 ? 2 : 1;
+        const int oi_self = (lane == own[0]) ? 0 : (lane == own[1]) ? 1 : -1;
+        if (oi_self >= 0) {
+            double* o = own_lds + 18 * oi_self;
+            for (int c = 0; c < 3; ++c)
+                for (int m = 0; m < 5; ++m) o[5 * c + m] = cf[c][m];
+            o[15] = s_end;
+            o[16] = pt;
+            o[17] = pph;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int w = rank; w < 4 * kConeSub * n_own; w += n_act) {
+            const int oi = w / (4 * kConeSub), j = w - oi * (4 * kConeSub);
+            const double* o = own_lds + 18 * oi;
+            double cf3[3][5];
+            for (int c = 0; c < 3; ++c)
+                for (int m = 0; m < 5; ++m) cf3[c][m] = o[5 * c + m];
+            JetNode n;
+            jet_node(F, cf3, o[15], o[16], o[17], j, n);
+            double* d = node_lds + 10 * w;
+            for (int m = 0; m < 7; ++m) d[m] = n.f[m];
+            for (int m = 0; m < 3; ++m) d[7 + m] = n.yp[m];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (oi_self >= 0) {
+            const double coef = wk * F.j0 * h;
+            double dJds = 0.0, sum = 0.0;
+            for (int j = 0; j < 4 * kConeSub; ++j) {
+                const double* d = node_lds + 10 * (oi_self * 4 * kConeSub + j);
+                JetNode n;
+                for (int m = 0; m < 7; ++m) n.f[m] = d[m];
+                for (int m = 0; m < 3; ++m) n.yp[m] = d[7 + m];
+                jet_accum(coef, s_end, j, n, cbar, cb, &sum, &dJds);
+            }
+            *Js = F.j0 * sum * h * s_end / kConeSub;
+            Os = coef * dJds;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    return Os;
+}
+#endif
 
 // Reverse of one frozen step from x with size h: the adjoint xb of its end
 // state (ignored for the terminal step, whose end state is not used) becomes
@@ -553,22 +672,32 @@ __host__ __device__ inline double jet_seeds(const Frame& F, const double cf[4][5
 // the kernel holds one copy of the reverse Dormand-Prince step.
 __host__ __device__ inline void step_reverse(const Frame& F, const RayK& K, const double x[4], double h,
                                              bool jet, bool term, double se, double Or, double wk, double xb[4],
-                                             double cb[3], double* J) {
+                                             double cb[3], double* J, double* lds = nullptr) {
     double ynb[4], kb[7][4] = {};
     for (int c = 0; c < 4; ++c) {
         ynb[c] = term ? 0.0 : xb[c];
         xb[c] = 0.0;
     }
-    const bool dense = term || (jet && wk != 0.0);
+    const bool need = jet && wk != 0.0;
+    const bool dense = term || need;
+    double cf[4][5], cbar[5][4] = {};
+    if (dense) step_dense(K, x, h, cf);
+    double Os = 0.0;
+#if defined(__HIP_DEVICE_COMPILE__)
+    // a few sampling lanes in the wave: every active lane takes part and the
+    // nodes are shared; many: each evaluates its own (the same bits either way)
+    if (lds && __popcll(__ballot(need)) <= kJetShareMax) {
+        double Js = 0.0;
+        Os = jet_seeds_wave(F, cf, h, term ? se : 1.0, wk, K.pt, K.pph, cbar, cb, &Js, need, lds);
+        if (need) *J += Js;
+    } else
+#endif
+    if (need) {
+        double Js = 0.0;
+        Os = jet_seeds(F, cf, h, term ? se : 1.0, wk, K.pt, K.pph, cbar, cb, &Js);
+        *J += Js;
+    }
     if (dense) {
-        double cf[4][5], cbar[5][4] = {};
-        step_dense(K, x, h, cf);
-        double Os = 0.0;
-        if (jet && wk != 0.0) {
-            double Js = 0.0;
-            Os = jet_seeds(F, cf, h, term ? se : 1.0, wk, K.pt, K.pph, cbar, cb, &Js);
-            *J += Js;
-        }
         if (term) {
             double rv, rd, tv, td;
             dense_vd(cf[0][0], cf[0][1], cf[0][2], cf[0][3], cf[0][4], se, rv, rd);
@@ -624,6 +753,7 @@ __host__ __device__ inline void fwd_init(const Frame& F, const RaygenArgs& G, in
     s.h = F.h0;
     s.pw = F.pw_init;
     s.jet = 0.0;
+    s.jet_wait = 0;
     s.nsteps = 0;
     s.tries = 0;
     write_ck(ck, s);
@@ -642,7 +772,11 @@ __host__ __device__ inline void dense4(const Ray& s, const Att& A, double cf[4][
 // jet integral accumulates in s.jet (render.cpp:107-122, 150-152: sampled on
 // every accepted step near the jet, over the part of the step before the
 // event that ends the ray).  Checkpoints go to ck every K accepted steps.
-__host__ __device__ inline int fwd_step(const Frame& F, Ray& s, double* ck, double yo[6], double* s_hit) {
+// With pend (a 17-double slot), a step's jet sample is not taken here: its
+// dense output (r, theta, p_r), s_end and h go to pend and s.jet_wait is set;
+// the caller adds j0 h s_end sum_j w_j jet_value_node(j) / kConeSub to s.jet.
+__host__ __device__ inline int fwd_step(const Frame& F, Ray& s, double* ck, double yo[6], double* s_hit,
+                                        double* pend = nullptr) {
     clamp_h(F, s);
     const RayK K = ray_k(F.a, F.a2, s.pt, s.pph);
     Att A;
@@ -680,7 +814,15 @@ __host__ __device__ inline int fwd_step(const Frame& F, Ray& s, double* ck, doub
         if (near_jet(F, s, A)) {
             double cf[4][5];
             dense4(s, A, cf);
-            s.jet += jet_step(F, cf, s.h, s_jet, s.pt, s.pph);
+            if (pend) {
+                for (int c = 0; c < 3; ++c)
+                    for (int m = 0; m < 5; ++m) pend[5 * c + m] = cf[c][m];
+                pend[15] = s_jet;
+                pend[16] = s.h;
+                s.jet_wait = 1;
+            } else {
+                s.jet += jet_step(F, cf, s.h, s_jet, s.pt, s.pph);
+            }
         }
         if (term < 0 && A.yn[1] > F.r_escape) {                       // render.cpp:247-251
             for (int c = 0; c < 6; ++c) yo[c] = A.yn[c];
@@ -759,7 +901,8 @@ __host__ __device__ inline bool same_ck(const Ray& s, const double* ck) {
 // number of replayed segments whose end state differed from the next
 // checkpoint (0 when the replay reproduces the forward exactly).
 __host__ __device__ inline int bwd_ray(const Frame& F, const RaygenArgs& G, int64_t ray, int n_acc, int cls,
-                                       double w, const double* ck, double* seg, int64_t stride, double grad[4]) {
+                                       double w, const double* ck, double* seg, int64_t stride, double grad[4],
+                                       double* lds = nullptr) {
     int mismatch = 0;
     int px, py, k;
     ray_pixel(G, ray, px, py, k);
@@ -830,7 +973,7 @@ __host__ __device__ inline int bwd_ray(const Frame& F, const RaygenArgs& G, int6
             const double* o = seg + static_cast<int64_t>(i) * kSeg * stride;
             const double xi[4] = {o[0], o[stride], o[2 * stride], o[3 * stride]};
             const bool term = disc && j == j_last && i == n_to - n_from - 1;
-            step_reverse(F, K, xi, o[4 * stride], o[5 * stride] != 0.0, term, se, Or, wk, xb, cb, &J);
+            step_reverse(F, K, xi, o[4 * stride], o[5 * stride] != 0.0, term, se, Or, wk, xb, cb, &J, lds);
         }
     }
     // camera: x0 = (r_cam, theta_cam, p_r, p_theta) and (p_t, p_phi) as
