@@ -374,7 +374,20 @@ def main():
         tr.shade_async(sc, rays, out)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    trace_ms = sum(a.elapsed_time(b) for a, b in ev) / args.steps
+    # device time of the whole trace call: counter fill, the cost-ordered
+    # queue's pilot + sort + scan, k_raygen and k_trace
+    trace_call_ms = sum(a.elapsed_time(b) for a, b in ev) / args.steps
+    # k_trace alone (and the queue ordering alone), from the HIP events the
+    # library records on the launch stream around each kernel; synchronous
+    # calls, so kept out of the timed loop above
+    reps = max(1, min(args.steps, 20))
+    kern, order = [], []
+    for _ in range(reps):
+        _, sk = tr.trace(sc, rays)
+        kern.append(sk.trace_ms)
+        order.append(sk.order_ms)
+    trace_ms = sum(kern) / reps
+    order_ms = sum(order) / reps
 
     ms_per_step = wall / args.steps * 1e3
     n_rays = sc.n_rays
@@ -403,6 +416,9 @@ def main():
         "accepted_steps_per_ray": st.steps / n_rays,
         "rejected_attempts": int(rejected),
         "trace_ms": round(trace_ms, 4),
+        "trace_call_ms": round(trace_call_ms, 4),
+        "queue_order_ms": round(order_ms, 4),
+        "queue_order": "cost" if order_ms > 0 else "centre-first",
         "shade_ms": round(st.shade_ms, 4),
         "batched_over_plain": None,
         "grad_over_fwd": None,

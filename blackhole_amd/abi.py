@@ -60,7 +60,7 @@ class Stats(C.Structure):
         + [(n, C.c_double) for n in ("max_norm_error", "max_carter_drift", "seconds",
                                      "total_flux")]
         + [("attempts", C.c_int64), ("rhs_evals", C.c_int64), ("trace_ms", C.c_double),
-           ("shade_ms", C.c_double)]
+           ("shade_ms", C.c_double), ("order_ms", C.c_double)]
     )
 
     def as_dict(self):
@@ -80,4 +80,7 @@ class Shaded(C.Structure):
 
 class Launch(C.Structure):
     _fields_ = [("block", C.c_int32), ("waves_per_simd", C.c_int32), ("chunk", C.c_int32),
-                ("refill", C.c_int32)]
+                ("refill", C.c_int32), ("queue_order", C.c_int32)]
+
+
+ORDER_CENTRE, ORDER_COST = 1, 2   # bhx_launch.queue_order

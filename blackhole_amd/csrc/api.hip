@@ -155,7 +155,9 @@ int bhx_destroy(bhx_context* ctx) {
     }
     for (DevBuf* b : {&ctx->cell_start, &ctx->star_rows,
                       &ctx->init, &ctx->q2ray, &ctx->evp, &ctx->flux_partial,
-                      &ctx->lum_hist, &ctx->lum_npos, &ctx->r_cls, &ctx->r_y, &ctx->r_jet, &ctx->r_xyz})
+                      &ctx->lum_hist, &ctx->lum_npos, &ctx->r_cls, &ctx->r_y, &ctx->r_jet, &ctx->r_xyz,
+                      &ctx->pilot_init, &ctx->pilot_q2ray, &ctx->pilot_att, &ctx->pilot_cls, &ctx->order_keys,
+                      &ctx->order_vals, &ctx->order_sizes, &ctx->cost_order, &ctx->cost_start, &ctx->order_tmp})
         b->release();
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -191,6 +193,9 @@ int bhx_set_launch(bhx_context* ctx, const bhx_launch* l) {
     if (l->refill < 0 || l->refill > 64) return fail(ctx, BHX_E_INVALID, "refill must be in [0, 64]");
     if (l->waves_per_simd < 0 || l->waves_per_simd > 8)
         return fail(ctx, BHX_E_INVALID, "waves_per_simd must be in [0, 8]");
+    if (l->queue_order < 0 || l->queue_order > BHX_ORDER_COST)
+        return fail(ctx, BHX_E_INVALID, "queue_order must be 0, BHX_ORDER_CENTRE or BHX_ORDER_COST");
+    if (l->queue_order) ctx->launch.queue_order = l->queue_order;
     if (l->block) ctx->launch.block = l->block;
     if (l->waves_per_simd) ctx->launch.waves_per_simd = l->waves_per_simd;
     if (l->chunk) ctx->launch.chunk = l->chunk;
@@ -257,6 +262,18 @@ int bhx_trace_dev(bhx_context* ctx, const bhx_scene* s, const bhx_rays* rays, bh
     if (rc) return rc;
     // counters are cleared by a fill kernel (never a memset node: the call may be captured)
     HIPCHK(launch_zero_u64(reinterpret_cast<unsigned long long*>(ctx->d_cnt), sizeof(Counters) / 8, st));
+    // longest-first queue when every resident lane sees several rays (below
+    // that the queue hardly drains before the end and the pilot is overhead)
+    const bool by_cost = ctx->launch.queue_order == BHX_ORDER_COST && !P.has_jet &&
+                         n_rays >= 4 * static_cast<int64_t>(grid) * block;
+    if (by_cost) {
+        HIPCHK(hipEventRecord(ctx->ev[4], st));
+        rc = cost_order(ctx, G, P, st);
+        if (rc) return rc;
+        HIPCHK(hipEventRecord(ctx->ev[5], st));
+        G.tile_order = static_cast<const int32_t*>(ctx->cost_order.p);
+        G.tile_start = static_cast<const int32_t*>(ctx->cost_start.p);
+    }
     HIPCHK(launch_raygen(G, st));
     HIPCHK(hipEventRecord(ctx->ev[0], st));
     HIPCHK(launch_trace(P, grid, block, st));
@@ -275,6 +292,7 @@ int bhx_trace_dev(bhx_context* ctx, const bhx_scene* s, const bhx_rays* rays, bh
         stats->max_norm_error = std::max(stats->max_norm_error, mn);
         stats->max_carter_drift = std::max(stats->max_carter_drift, mc);
         stats->trace_ms = ms_between(ctx->ev[0], ctx->ev[1]);
+        stats->order_ms = by_cost ? ms_between(ctx->ev[4], ctx->ev[5]) : 0.0;
     }
     return BHX_OK;
 }

@@ -16,6 +16,7 @@ namespace bhx {
 hipError_t launch_raygen(const RaygenArgs& args, hipStream_t stream);
 hipError_t launch_trace(const TraceArgs& args, int grid, int block, hipStream_t stream);
 hipError_t trace_occupancy(int block, int* blocks_per_cu);
+hipError_t launch_pilot(const RaygenArgs& gen, const TraceArgs& args, int block, hipStream_t stream);
 hipError_t launch_trace_batch(const TraceArgs& args, int grid, int block, hipStream_t stream);
 hipError_t trace_batch_occupancy(int block, int n_scenes, int* blocks_per_cu);
 hipError_t launch_shade(const ShadeArgs& args, int blocks, hipStream_t s);
@@ -33,7 +34,8 @@ struct Counters {
     unsigned long long totals[4];       // steps, attempts, max norm bits, max Carter bits
     unsigned long long classes[8];      // 6 classes + jet rays
     unsigned int tile_counter;
-    unsigned int pad;
+    unsigned int pilot_counter;          // queue of the cost-order pilot
+    unsigned long long pilot_totals[4];  // the pilot's totals (not reported)
 };
 
 struct DevBuf {
@@ -59,7 +61,7 @@ struct bhx_context {
     int device = 0;
     int n_cu = 0;
     std::string err;
-    bhx_launch launch{256, 0, 32, 8};   // chunk 32, refill 8: fastest of a sweep on the 1024² frame
+    bhx_launch launch{256, 0, 32, 8, BHX_ORDER_COST};   // chunk 32, refill 8: fastest of a sweep on the 1024² frame
     // tables
     double* d_bb = nullptr;
     // catalogue (cell CSR + star rows)
@@ -79,6 +81,11 @@ struct bhx_context {
     };
     std::map<std::tuple<int, int, int, bool, double>, TileSet> tiles;   // (W, H, S, axis band, roll)
     const TileSet* cur_tiles = nullptr;
+    // cost-ordered queue of the last frame: pilot rays (init rows, slot map,
+    // attempts, classes), sort keys/values, tile sizes, the sorted tile order
+    // and its queue offsets, and the sort/scan workspace
+    DevBuf pilot_init, pilot_q2ray, pilot_att, pilot_cls, order_keys, order_vals, order_sizes;
+    DevBuf cost_order, cost_start, order_tmp;
     // initial ray states in queue order (kInitRows x n) and slot -> ray map
     DevBuf init, q2ray;
     // jet / Sun parameters of the tracer's event branches (last uploaded copy)
@@ -92,8 +99,11 @@ struct bhx_context {
     DevBuf r_cls, r_y, r_jet, r_xyz;
     // veiling glare: row pass, the two blurred images, the taps
     DevBuf glare_tmp, glare_near, glare_far, glare_taps;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 };
+
+// cost-ordered queue of the current frame into ctx->cost_order / cost_start (order.hip)
+int cost_order(bhx_context* ctx, const bhx::RaygenArgs& G, const bhx::TraceArgs& P, hipStream_t st);
 
 inline int bhx_fail(bhx_context* ctx, int code, const std::string& msg) {
     if (ctx) ctx->err = msg;

@@ -50,6 +50,14 @@ struct SceneK {
 static_assert(sizeof(SceneK) % sizeof(double) == 0, "SceneK is copied as doubles");
 constexpr int kMaxBatchScenes = 128;
 
+// Pilot of the cost-ordered queue (one coarse ray per 8 x 8 tile): step-size
+// cap kPilotHmax times the reference's hmax (render.cpp:130-131), at least
+// these tolerances, at most kPilotSteps accepted steps.  Its attempt counts
+// only rank the tiles.
+constexpr double kPilotHmax = 16.0;
+constexpr double kPilotRtol = 1e-4, kPilotAtol = 1e-6;
+constexpr int kPilotSteps = 32;
+
 // Ray generation: camera, jitter and the initial photon of every sample
 // (src/render.cpp:437-452, src/scene.h:356-384, src/geodesic.h:215-231),
 // written in work-queue order so the tracer's refills are coalesced loads.

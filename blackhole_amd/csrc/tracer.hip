@@ -58,6 +58,35 @@ __device__ __forceinline__ double uniform_d(double x) {
 // ---------------------------------------------------------------------------
 // Ray generation: one block of 64 threads per queue tile.
 // ---------------------------------------------------------------------------
+// Sample k of pixel (px, py): camera, photon, Carter Q0 and the first RHS,
+// written to queue slot q of the SoA initial-state rows (row stride N).
+__device__ __forceinline__ void gen_sample(const RaygenArgs& P, int px, int py, int k, int q, int ray,
+                                           int64_t N) {
+    double d[3];
+    camera_dir(P, px, py, k, d);
+    double pt, pr, pth, pph;
+    camera_photon(P, d, pt, pr, pth, pph);
+    // Carter Q at the camera (kerr.h:342-348)
+    const double E = -pt;
+    const double q0 = pth * pth + P.cam_cs2 * (P.a2 * (0.0 - E * E) + pph * pph / P.cam_sn2);
+    Rhs f;
+    kerr_rhs(ray_k(P.a, P.a2, pt, pph), P.cam_r, P.cam_th, pr, pth, f);
+    double* I = P.init;
+    I[kI_T * N + q] = 0.0;
+    I[kI_R * N + q] = P.cam_r;
+    I[kI_TH * N + q] = P.cam_th;
+    I[kI_PH * N + q] = P.cam_ph;
+    I[kI_PR * N + q] = pr;
+    I[kI_PTH * N + q] = pth;
+    I[kI_PT * N + q] = pt;
+    I[kI_PPH * N + q] = pph;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) I[(kI_K1 + c) * N + q] = f.d[c];
+    I[kI_C0 * N + q] = f.cs;
+    I[kI_Q0 * N + q] = q0;
+    P.q2ray[q] = ray;
+}
+
 __global__ void __launch_bounds__(64) k_raygen(const RaygenArgs P) {
     const int t = blockIdx.x;
     if (t >= P.n_tiles) return;
@@ -73,32 +102,20 @@ __global__ void __launch_bounds__(64) k_raygen(const RaygenArgs P) {
         const int pin = i / S2, k = i - pin * S2;
         const int px = x0 + pin % tw, py = y0 + pin / tw;
         const int q = base + i;
-        const int ray = P.ray_base + (py * P.W + px) * S2 + k;
-        double d[3];
-        camera_dir(P, px, py, k, d);
-        double pt, pr, pth, pph;
-        camera_photon(P, d, pt, pr, pth, pph);
-        // Carter Q at the camera (kerr.h:342-348)
-        const double E = -pt;
-        const double q0 = pth * pth + P.cam_cs2 * (P.a2 * (0.0 - E * E) + pph * pph / P.cam_sn2);
-        Rhs f;
-        kerr_rhs(ray_k(P.a, P.a2, pt, pph), P.cam_r, P.cam_th, pr, pth, f);
-        double* I = P.init;
-        I[kI_T * N + q] = 0.0;
-        I[kI_R * N + q] = P.cam_r;
-        I[kI_TH * N + q] = P.cam_th;
-        I[kI_PH * N + q] = P.cam_ph;
-        I[kI_PR * N + q] = pr;
-        I[kI_PTH * N + q] = pth;
-        I[kI_PT * N + q] = pt;
-        I[kI_PPH * N + q] = pph;
-#pragma unroll
-        for (int c = 0; c < 6; ++c) I[(kI_K1 + c) * N + q] = f.d[c];
-        I[kI_C0 * N + q] = f.cs;
-        I[kI_Q0 * N + q] = q0;
-        P.q2ray[q] = ray;
+        gen_sample(P, px, py, k, q, P.ray_base + (py * P.W + px) * S2 + k, N);
         if (P.q2scene) P.q2scene[q] = (uint8_t)P.scene_id;
     }
+}
+
+// Pilot rays of the cost-ordered queue: one per 8 x 8 tile, at the tile's
+// centre pixel (clamped into a partial edge tile), first sample; slot = ray =
+// row-major tile index, row stride n_tiles.
+__global__ void __launch_bounds__(256) k_pilot_gen(const RaygenArgs P) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P.n_tiles) return;
+    const int tx = t % P.tiles_x, ty = t / P.tiles_x;
+    const int px = min(tx * kTile + kTile / 2, P.W - 1), py = min(ty * kTile + kTile / 2, P.H - 1);
+    gen_sample(P, px, py, 0, t, t, P.n_tiles);
 }
 
 // ---------------------------------------------------------------------------
@@ -111,7 +128,7 @@ __global__ void __launch_bounds__(64) k_raygen(const RaygenArgs P) {
 // LDS copy `sk` of the scene table).
 #define SK(f) (kBatch ? sk[sid].f : P.f)
 #define EV (kBatch ? sk[sid].ev : *P.ev)
-template <bool kBatch>
+template <bool kBatch, bool kPilot = false>
 __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk, double* pend,
                                            unsigned long long* blk) {
     // ---- per-scene constants -------------------------------------------------
@@ -314,7 +331,8 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
 
         // ---- one DP5 attempt (geodesic.h:130-194) ---------------------------
         {
-            const double hmax = 0.05 * dmax(1.0, y[1] - SK(r_h)) + 0.01;
+            double hmax = 0.05 * dmax(1.0, y[1] - SK(r_h)) + 0.01;
+            if constexpr (kPilot) hmax *= kPilotHmax;   // the pilot's coarse steps
             h = (tries == 0 && h > hmax) ? hmax : h;   // clamp once per step, not per retry
         }
         double tmp[6], k2[6], k3[6], k4[6], k5[6], k6[6], yn[6];
@@ -608,6 +626,18 @@ k_trace(const TraceArgs P) {
     flush_totals(P, blk);
 }
 
+// Pilot of the cost-ordered queue: the same tracer with the coarse hmax, at
+// the tolerances and step cap its arguments carry.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_trace_pilot(const TraceArgs P) {
+    __shared__ unsigned long long blk[4];
+    extern __shared__ double pend_raw[];
+    if (threadIdx.x < 4) blk[threadIdx.x] = 0ull;
+    __syncthreads();
+    trace_body<false, true>(P, nullptr, pend_raw, blk);
+    flush_totals(P, blk);
+}
+
 // Batched: the scene table (n_scenes rows) is copied into dynamic LDS first.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 k_trace_batch(const TraceArgs P) {
@@ -638,6 +668,17 @@ static size_t pend_bytes(int block) { return sizeof(double) * kPend * (size_t)bl
 
 hipError_t launch_trace(const TraceArgs& args, int grid, int block, hipStream_t stream) {
     hipLaunchKernelGGL(k_trace, dim3(grid), dim3(block), pend_bytes(block), stream, args);
+    return hipGetLastError();
+}
+
+hipError_t launch_pilot(const RaygenArgs& gen, const TraceArgs& args, int block, hipStream_t stream) {
+    hipLaunchKernelGGL(k_pilot_gen, dim3((gen.n_tiles + 255) / 256), dim3(256), 0, stream, gen);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // every wave takes one chunk of 64 pilot rays: n_tiles / 64 waves in all
+    const int waves = (gen.n_tiles + 63) / 64;
+    const int grid = (waves * 64 + block - 1) / block;
+    hipLaunchKernelGGL(k_trace_pilot, dim3(grid), dim3(block), pend_bytes(block), stream, args);
     return hipGetLastError();
 }
 

@@ -65,15 +65,17 @@ class Tracer:
         native.check(self.ctx, self.lib.bhx_set_catalogue(
             self.ctx, len(arrs[0]), *[a.ctypes.data_as(C.c_void_p) for a in arrs]), "bhx_set_catalogue")
 
-    def set_launch(self, block=0, waves_per_simd=0, chunk=0, refill=0):
+    def set_launch(self, block=0, waves_per_simd=0, chunk=0, refill=0, queue_order=0):
+        """Launch shape of single-frame traces (0 keeps a field); queue_order is
+        abi.ORDER_CENTRE or abi.ORDER_COST (the default)."""
         native.check(self.ctx, self.lib.bhx_set_launch(
-            self.ctx, C.byref(Launch(block, waves_per_simd, chunk, refill))), "bhx_set_launch")
+            self.ctx, C.byref(Launch(block, waves_per_simd, chunk, refill, queue_order))), "bhx_set_launch")
 
     def get_launch(self):
         l = Launch()
         native.check(self.ctx, self.lib.bhx_get_launch(self.ctx, C.byref(l)), "bhx_get_launch")
         return {"block": l.block, "waves_per_simd": l.waves_per_simd, "chunk": l.chunk,
-                "refill": l.refill}
+                "refill": l.refill, "queue_order": l.queue_order}
 
     # ---- drop-in frame API ---------------------------------------------------
     def render(self, scene):

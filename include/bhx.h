@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define BHX_ABI_VERSION 2
+#define BHX_ABI_VERSION 3
 
 /* Error codes. */
 #define BHX_OK 0
@@ -115,6 +115,8 @@ typedef struct bhx_stats {
     int64_t rhs_evals;        /* geodesic_rhs evaluations on the GPU          */
     double trace_ms;          /* device time of the trace kernel (HIP events) */
     double shade_ms;          /* device time of the shading kernel            */
+    double order_ms;          /* device time of the cost-ordered queue's pilot,
+                                 sort and scan (0 with the centre-first order) */
 } bhx_stats;
 
 /* Per-ray outputs of the tracer (device pointers; any may be NULL). */
@@ -219,7 +221,20 @@ typedef struct bhx_launch {
     int32_t waves_per_simd;   /* resident waves per SIMD the grid is sized for */
     int32_t chunk;            /* rays a wave takes from the queue at once     */
     int32_t refill;           /* refill when at least this many lanes idle    */
+    int32_t queue_order;      /* single frames: BHX_ORDER_CENTRE or
+                                 BHX_ORDER_COST (the default; frames of at
+                                 least 4 rays per resident lane without a
+                                 jet); batches ignore it                      */
 } bhx_launch;
+
+/* Work-queue order of a single frame.  Both trace the same rays to the same
+ * bits; they differ only in when each 8 x 8 tile starts.
+ *   CENTRE  tiles by distance from the image centre;
+ *   COST    tiles longest-first, ranked by a pilot trace of one coarse ray
+ *           per tile (rtol 1e-4, 16x the step-size cap, at most 32 steps)
+ *           and the largest pilot cost in the tile's 3 x 3 neighbourhood, so
+ *           the frame ends on its shortest rays. */
+enum { BHX_ORDER_CENTRE = 1, BHX_ORDER_COST = 2 };
 
 int bhx_set_launch(bhx_context *ctx, const bhx_launch *launch);
 int bhx_get_launch(const bhx_context *ctx, bhx_launch *launch);

@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     assert set(native.SYMBOLS) <= set(names)
-    assert lib.bhx_abi_version() == 2
+    assert lib.bhx_abi_version() == 3
 
 
 def test_no_gpu_call_needed_for_errors():
