@@ -56,11 +56,16 @@ __host__ __device__ __forceinline__ void sincos_near(double x, double* s, double
     const double hz = 0.5 * z;
     const double w = 1.0 - hz;
     const double cr = w + (((1.0 - w) - hz) + z * z * pc);
-    const int q = static_cast<int>(k) & 3;
-    const double a = (q & 1) ? cr : sr;   // sin for q = 0, 2; cos for q = 1, 3
-    const double b = (q & 1) ? sr : cr;
-    *s = (q & 2) ? -a : a;
-    *c = ((q + 1) & 2) ? -b : b;
+    // x = r + q pi/2: (sin x, cos x) = (sr, cr), (cr, -sr), (-sr, -cr), (-cr, sr)
+    // for q = 0..3 (mod 4): pick by the low bit, then flip both signs by bit 1
+    // (an exclusive or of the sign bit, no selects)
+    const int q = static_cast<int>(k);
+    const bool odd = (q & 1) != 0;
+    const double a = odd ? cr : sr;
+    const double b = odd ? -sr : cr;
+    const uint64_t flip = static_cast<uint64_t>(q & 2) << 62;
+    *s = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, a) ^ flip);
+    *c = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, b) ^ flip);
 }
 
 // exp(x) for |x| < 700 to <= 1 ulp (checked against glibc): Cody-Waite
@@ -147,7 +152,9 @@ struct Rhs {
 // The same with sin and cos of theta given (sn, cs).
 __host__ __device__ __forceinline__ void kerr_rhs_sc(const RayK& K, double r, double sn, double cs, double pr,
                                                      double pth, Rhs& o) {
-    if (fabs(sn) < 1e-9) sn = (sn < 0.0) ? -1e-9 : 1e-9;  // kerr.h:34,51
+    // |sin theta| >= 1e-9 with its sign kept (kerr.h:34,51), as one max and a
+    // sign copy; a NaN angle gives +-1e-9 here but its cosine stays NaN
+    sn = copysign(fmax(fabs(sn), 1e-9), sn);
     const double sn2 = sn * sn;
     const double r2 = r * r;
     const double rpa = r2 + K.a2;

@@ -140,7 +140,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
     int q = 0;                          // queue slot of the lane's ray
     double y[6], k1[6];                 // t, r, th, ph, p_r, p_th
     double pt = 0.0, pph = 0.0;         // conserved p_t, p_phi
-    double h = 0.0, pw = 0.0, c_prev = 0.0, q0raw = 0.0, iq0 = 0.0;
+    double h = 0.0, lpw = 0.0, c_prev = 0.0, q0raw = 0.0, iq0 = 0.0;   // lpw = beta log(err_prev)
     double mx_norm = 0.0, mx_carter = 0.0, jet_acc = 0.0;
     int nsteps = 0, natt = 0, tries = 0;
     // conical jet: a sample of the last step waits (jet_pend) in the lane's LDS
@@ -300,7 +300,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
                     q0raw = I[kI_Q0 * N + q];
                     iq0 = rcp(fabs(q0raw) + 1e-12);
                     h = SK(h0);
-                    pw = P.pw_init;
+                    lpw = kBeta * kLog1em4;   // err_prev = 1e-4 (geodesic.h:197)
                     nsteps = natt = tries = 0;
                     mx_norm = mx_carter = jet_acc = 0.0;
                     live = SK(max_steps) > 0;
@@ -369,28 +369,37 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
         }
         Rhs g7;
         kerr_rhs(K, yn[1], yn[2], yn[4], yn[5], g7);
-        double err2 = 0.0, c5[6];
+        double err2 = 0.0;
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             const double e = h * (E1 * k1[c] + E3 * k3[c] + E4 * k4[c] + E5 * k5[c] + E6 * k6[c] +
                                   E7 * g7.d[c]);
-            const double sc = latol + lrtol * dmax(fabs(y[c]), fabs(yn[c]));
+            // max of two magnitudes: y is an accepted state, so never NaN
+            const double sc = latol + lrtol * fmax(fabs(y[c]), fabs(yn[c]));
             const double qe = e * rcp(sc);
             err2 += qe * qe;   // the p_t, p_phi terms are exactly 0
-            c5[c] = h * (D1 * k1[c] + D3 * k3[c] + D4 * k4[c] + D5 * k5[c] + D6 * k6[c] +
-                         D7 * g7.d[c]);
         }
-        double err = sqrt(err2 * 0.125);
-        if (!isfinite(err)) err = 1e10;
-        // fac = 0.9 err^-alpha err_prev^beta, as exp(-alpha log err) * pw
-        const double lg = log_pos(dmax(err, 1e-16));
-        double fac = 0.9 * exp_near(-kAlpha * lg) * pw;
+        // The RMS error is err = sqrt(u), u = err2 / 8 (geodesic.h:157-166),
+        // used only through err <= 1 and log err, so the square root is never
+        // taken: sqrt(u) <= 1 exactly when u <= 1 + 2^-52 (sqrt rounds 1 + 2^-52
+        // to 1), and log err = log(u) / 2.  A non-finite u stands for err = 1e10.
+        double u = err2 * 0.125;
+        if (!isfinite(u)) u = 1e20;
+        // fac = 0.9 err^-alpha err_prev^beta (geodesic.h:170-173) as one
+        // exponential, exp(beta log err_prev - alpha log err)
+        const double lg = 0.5 * log_pos(dmax(u, 1e-16 * 1e-16));
+        double fac = 0.9 * exp_near(lpw - kAlpha * lg);
         fac = dclamp(fac, 0.2, 5.0);
         ++natt;
+        // Hairer's fifth dense-output coefficient of component c, built only on
+        // the steps that locate an event or sample the jet
+        auto dense_c5 = [&](int c) {
+            return h * (D1 * k1[c] + D3 * k3[c] + D4 * k4[c] + D5 * k5[c] + D6 * k6[c] + D7 * g7.d[c]);
+        };
 
         int term = -1;  // class once the ray ends
         double yo[6];
-        if (err <= 1.0) {
+        if (u <= 1.0 + 0x1p-52) {
             // ---- accepted: invariants (render.cpp:140-145) -------------------
             ++nsteps;
             tries = 0;
@@ -425,7 +434,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
                 if (near_jet && EVS(model) == 1) {
                     const double c2t = yn[2] - y[2];
                     const double c3t = h * k1[2] - c2t;
-                    near_jet = cone_reach(y[2], c2t, c3t, c2t - h * g7.d[2] - c3t, c5[2], EVS(cone_angle));
+                    near_jet = cone_reach(y[2], c2t, c3t, c2t - h * g7.d[2] - c3t, dense_c5(2), EVS(cone_angle));
                 }
                 if (SK(has_sun) && !ns_hit && !horizon) {                             // render.cpp:203-209
                     const double e0 = p1[0] - p0[0], e1 = p1[1] - p0[1], e2 = p1[2] - p0[2];
@@ -435,6 +444,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
                 }
             }
             double s_jet = -1.0;   // fraction of the step the jet integral covers
+            double c5[6];          // set when the event branch below runs
             if (horizon) {
                 // ---- horizon (render.cpp:172-177): after the NS test, before the disc
 #pragma unroll
@@ -451,6 +461,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
                     c2[c] = yn[c] - y[c];
                     c3[c] = h * k1[c] - c2[c];
                     c4[c] = c2[c] - h * g7.d[c] - c3[c];
+                    c5[c] = dense_c5(c);
                 }
                 if (ns_hit) {
                     // ---- neutron-star surface (render.cpp:158-169) -------------
@@ -560,7 +571,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
 #pragma unroll
                 for (int c = 0; c < 6; ++c) { y[c] = yn[c]; k1[c] = g7.d[c]; }
                 c_prev = g7.cs;
-                pw = exp_near(kBeta * dmax(lg, kLog1em4));   // pow(max(err, 1e-4), beta)
+                lpw = kBeta * dmax(lg, kLog1em4);   // err_prev = max(err, 1e-4)
                 h *= fac;
                 if (nsteps >= SK(max_steps)) {
 #pragma unroll
