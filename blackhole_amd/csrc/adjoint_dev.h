@@ -35,12 +35,12 @@ using namespace dp5;
 
 constexpr double kPi_ = 3.14159265358979323846;
 constexpr double kSigmaSB_ = 5.670374419e-8;
-constexpr int kCk = 8;   // checkpoint record: y[6], h, controller memory
+constexpr int kCk = 8;   // checkpoint record: y[6], h, controller memory (beta log err_prev)
 
 // Per-frame constants of a differentiable render.
 struct Frame {
     // integrator (src/render.cpp:71-86)
-    double a, a2, r_h, r_stop, rtol, atol, r_escape, h0, pw_init, r_in, r_out;
+    double a, a2, r_h, r_stop, rtol, atol, r_escape, h0, r_in, r_out;
     int32_t max_steps, K, cap, sg;
     // Novikov-Thorne disc (src/disc.h:46-121) and the spin derivatives of its
     // spin-only constants
@@ -59,7 +59,7 @@ struct Frame {
 
 // Integrator state of one ray.
 struct Ray {
-    double y[6], k1[6], pt, pph, h, pw, c_prev, jet;
+    double y[6], k1[6], pt, pph, h, lpw, c_prev, jet;   // lpw = beta log(err_prev)
     int nsteps, tries;
     int jet_wait;   // a jet sample of the last step waits in the caller's slot (see fwd_step)
 };
@@ -67,12 +67,14 @@ struct Ray {
 __host__ __device__ inline void write_ck(double* ck, const Ray& s) {
     for (int c = 0; c < 6; ++c) ck[c] = s.y[c];
     ck[6] = s.h;
-    ck[7] = s.pw;
+    ck[7] = s.lpw;
 }
 
 // One DP5 attempt from s (the tracer's arithmetic, src/geodesic.h:130-169).
+// u = err^2 (err the RMS scaled error; 1e20 when not finite): the square root
+// is never taken, see accepted() and control_fac().
 struct Att {
-    double yn[6], c5[6], err;
+    double yn[6], c5[6], u;
     Rhs g7;
 };
 __host__ __device__ inline void attempt(const RayK& K, const Ray& s, double rtol, double atol, Att& o) {
@@ -108,15 +110,18 @@ __host__ __device__ inline void attempt(const RayK& K, const Ray& s, double rtol
     double err2 = 0.0;
     for (int c = 0; c < 6; ++c) {
         const double e = h * (E1 * k1[c] + E3 * k3[c] + E4 * k4[c] + E5 * k5[c] + E6 * k6[c] + E7 * o.g7.d[c]);
-        const double sc = atol + rtol * dmax(fabs(y[c]), fabs(o.yn[c]));
+        const double sc = atol + rtol * fmax(fabs(y[c]), fabs(o.yn[c]));   // y is never NaN
         const double qe = e * rcp(sc);
         err2 += qe * qe;
         o.c5[c] = h * (D1 * k1[c] + D3 * k3[c] + D4 * k4[c] + D5 * k5[c] + D6 * k6[c] + D7 * o.g7.d[c]);
     }
-    double err = sqrt(err2 * 0.125);
-    if (!finite_d(err)) err = 1e10;
-    o.err = err;
+    double u = err2 * 0.125;
+    if (!finite_d(u)) u = 1e20;   // err = 1e10 (geodesic.h:166)
+    o.u = u;
 }
+
+// err <= 1 (geodesic.h:175): sqrt(u) <= 1 exactly when u <= 1 + 2^-52.
+__host__ __device__ inline bool accepted(const Att& A) { return A.u <= 1.0 + 0x1p-52; }
 
 // hmax clamp before a step (src/render.cpp:130-131), once per step.
 __host__ __device__ inline void clamp_h(const Frame& F, Ray& s) {
@@ -124,10 +129,11 @@ __host__ __device__ inline void clamp_h(const Frame& F, Ray& s) {
     if (s.tries == 0 && s.h > hmax) s.h = hmax;
 }
 
-// PI controller factor (src/geodesic.h:170-173) and the log of the error.
-__host__ __device__ inline double control_fac(const Ray& s, double err, double* lg_out) {
-    const double lg = log_pos(dmax(err, 1e-16));
-    double fac = 0.9 * exp_near(-kAlpha * lg) * s.pw;
+// PI controller factor (src/geodesic.h:170-173), 0.9 err^-alpha
+// err_prev^beta as one exponential, and the log of the error (log(u) / 2).
+__host__ __device__ inline double control_fac(const Ray& s, double u, double* lg_out) {
+    const double lg = 0.5 * log_pos(dmax(u, 1e-16 * 1e-16));
+    double fac = 0.9 * exp_near(s.lpw - kAlpha * lg);
     *lg_out = lg;
     return dclamp(fac, 0.2, 5.0);
 }
@@ -136,7 +142,7 @@ __host__ __device__ inline double control_fac(const Ray& s, double err, double* 
 __host__ __device__ inline void advance(Ray& s, const Att& A, double fac, double lg) {
     for (int c = 0; c < 6; ++c) { s.y[c] = A.yn[c]; s.k1[c] = A.g7.d[c]; }
     s.c_prev = A.g7.cs;
-    s.pw = exp_near(kBeta * dmax(lg, kLog1em4));
+    s.lpw = kBeta * dmax(lg, kLog1em4);   // err_prev = max(err, 1e-4)
     s.h *= fac;
 }
 
@@ -751,7 +757,7 @@ __host__ __device__ inline void fwd_init(const Frame& F, const RaygenArgs& G, in
     for (int c = 0; c < 6; ++c) s.k1[c] = f.d[c];
     s.c_prev = f.cs;
     s.h = F.h0;
-    s.pw = F.pw_init;
+    s.lpw = kBeta * kLog1em4;   // err_prev = 1e-4 (geodesic.h:197)
     s.jet = 0.0;
     s.jet_wait = 0;
     s.nsteps = 0;
@@ -782,8 +788,8 @@ __host__ __device__ inline int fwd_step(const Frame& F, Ray& s, double* ck, doub
     Att A;
     attempt(K, s, F.rtol, F.atol, A);
     double lg;
-    const double fac = control_fac(s, A.err, &lg);
-    if (A.err <= 1.0) {
+    const double fac = control_fac(s, A.u, &lg);
+    if (accepted(A)) {
         ++s.nsteps;
         s.tries = 0;
         int term = -1;
@@ -860,7 +866,7 @@ __host__ __device__ inline void replay(const Frame& F, Ray& s, const double* ck,
                                        int64_t stride) {
     for (int c = 0; c < 6; ++c) s.y[c] = ck[c];
     s.h = ck[6];
-    s.pw = ck[7];
+    s.lpw = ck[7];
     s.nsteps = n_from;
     s.tries = 0;
     const RayK K = ray_k(F.a, F.a2, s.pt, s.pph);
@@ -873,8 +879,8 @@ __host__ __device__ inline void replay(const Frame& F, Ray& s, const double* ck,
         Att A;
         attempt(K, s, F.rtol, F.atol, A);
         double lg;
-        const double fac = control_fac(s, A.err, &lg);
-        if (A.err <= 1.0) {
+        const double fac = control_fac(s, A.u, &lg);
+        if (accepted(A)) {
             double* o = seg + static_cast<int64_t>(s.nsteps - n_from) * kSeg * stride;
             o[0] = s.y[1]; o[stride] = s.y[2]; o[2 * stride] = s.y[4]; o[3 * stride] = s.y[5];
             o[4 * stride] = s.h;
@@ -892,7 +898,7 @@ __host__ __device__ inline void replay(const Frame& F, Ray& s, const double* ck,
 __host__ __device__ inline bool same_ck(const Ray& s, const double* ck) {
     for (int c = 0; c < 6; ++c)
         if (s.y[c] != ck[c]) return false;
-    return s.h == ck[6] && s.pw == ck[7];
+    return s.h == ck[6] && s.lpw == ck[7];
 }
 
 // Adjoint of w * I for one ray that ended after n_acc accepted steps with
@@ -934,8 +940,8 @@ __host__ __device__ inline int bwd_ray(const Frame& F, const RaygenArgs& G, int6
             Att A;
             attempt(K, s, F.rtol, F.atol, A);
             double lg;
-            const double fac = control_fac(s, A.err, &lg);
-            if (A.err <= 1.0) {
+            const double fac = control_fac(s, A.u, &lg);
+            if (accepted(A)) {
                 double cr[5], ct[5];
                 dense_coeffs(s, A, 1, cr);
                 dense_coeffs(s, A, 2, ct);

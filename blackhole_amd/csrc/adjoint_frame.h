@@ -41,7 +41,7 @@ inline bool make_frame(const bhx_scene& s, int K, Frame& F, RaygenArgs& G) {
     const TraceArgs T = host::trace_args(s);
     const ShadeArgs S = host::shade_args(s);
     F.a = T.a; F.a2 = T.a2; F.r_h = T.r_h; F.r_stop = T.r_stop; F.rtol = T.rtol; F.atol = T.atol;
-    F.r_escape = T.r_escape; F.h0 = T.h0; F.pw_init = T.pw_init; F.r_in = T.r_in; F.r_out = T.r_out;
+    F.r_escape = T.r_escape; F.h0 = T.h0; F.r_in = T.r_in; F.r_out = T.r_out;
     F.max_steps = T.max_steps;
     F.K = K;
     F.cap = T.max_steps / K + 1;

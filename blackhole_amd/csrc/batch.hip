@@ -238,7 +238,6 @@ int bhx_batch_create(bhx_context* ctx, const bhx_scene* scenes, int32_t n, bhx_b
         k.has_disc = T.has_disc; k.has_ns = T.has_ns; k.has_jet = T.has_jet; k.has_sun = T.has_sun;
         k.max_steps = T.max_steps;
         k.ev = host::event_params(s);
-        if (i == 0) b->trace.pw_init = T.pw_init;
 
         RaygenArgs G = host::raygen_args(s);
         G.tiles_x = (s.width + 7) / 8;

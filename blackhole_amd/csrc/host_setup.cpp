@@ -311,7 +311,6 @@ TraceArgs trace_args(const bhx_scene& s) {
     P.atol = s.atol;
     P.r_escape = s.r_escape;
     P.h0 = 0.02 * std::max(1.0, s.cam_r);                              // render.cpp:86
-    P.pw_init = std::pow(std::max(1e-4, 1e-4), 0.4 / 5.0);             // geodesic.h:171-172
     P.max_steps = s.max_steps;
     P.has_disc = s.disc_enabled != 0;
     if (P.has_disc) {  // NovikovThorneDisc ctor (disc.h:73-82)

@@ -93,7 +93,6 @@ enum InitRow { kI_T = 0, kI_R, kI_TH, kI_PH, kI_PR, kI_PTH, kI_PT, kI_PPH,
 struct TraceArgs {
     // metric and integrator (src/render.cpp:71-86, src/render.h:45-48)
     double a, a2, r_h, r_stop, rtol, atol, r_escape, h0;
-    double pw_init;                 // pow(1e-4, 0.08): controller memory of a fresh ray
     // events (src/render.cpp:158-199)
     double r_in, r_out, ns_R;
     int32_t has_disc, has_ns, has_jet, has_sun, max_steps, refill, chunk;
