@@ -115,9 +115,9 @@ __host__ __device__ inline void attempt(const RayK& K, const Ray& s, double rtol
         err2 += qe * qe;
         o.c5[c] = h * (D1 * k1[c] + D3 * k3[c] + D4 * k4[c] + D5 * k5[c] + D6 * k6[c] + D7 * o.g7.d[c]);
     }
-    double u = err2 * 0.125;
-    if (!finite_d(u)) u = 1e20;   // err = 1e10 (geodesic.h:166)
-    o.u = u;
+    // err = 1e10 when not finite (geodesic.h:166): min(u, 1e20) maps inf and
+    // NaN there, and any u >= 1e20 is rejected with fac clamped to 0.2 alike
+    o.u = fmin(err2 * 0.125, 1e20);
 }
 
 // err <= 1 (geodesic.h:175): sqrt(u) <= 1 exactly when u <= 1 + 2^-52.
@@ -132,10 +132,9 @@ __host__ __device__ inline void clamp_h(const Frame& F, Ray& s) {
 // PI controller factor (src/geodesic.h:170-173), 0.9 err^-alpha
 // err_prev^beta as one exponential, and the log of the error (log(u) / 2).
 __host__ __device__ inline double control_fac(const Ray& s, double u, double* lg_out) {
-    const double lg = 0.5 * log_pos(dmax(u, 1e-16 * 1e-16));
-    double fac = 0.9 * exp_near(s.lpw - kAlpha * lg);
+    const double lg = 0.5 * log_pos(fmax(u, 1e-16 * 1e-16));
     *lg_out = lg;
-    return dclamp(fac, 0.2, 5.0);
+    return fmin(fmax(0.9 * exp_near(s.lpw - kAlpha * lg), 0.2), 5.0);   // never NaN: std::clamp
 }
 
 // Accepted step that does not end the ray: advance the state.

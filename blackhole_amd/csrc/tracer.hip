@@ -382,14 +382,15 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
         // The RMS error is err = sqrt(u), u = err2 / 8 (geodesic.h:157-166),
         // used only through err <= 1 and log err, so the square root is never
         // taken: sqrt(u) <= 1 exactly when u <= 1 + 2^-52 (sqrt rounds 1 + 2^-52
-        // to 1), and log err = log(u) / 2.  A non-finite u stands for err = 1e10.
-        double u = err2 * 0.125;
-        if (!isfinite(u)) u = 1e20;
+        // to 1), and log err = log(u) / 2.  A non-finite u stands for err = 1e10
+        // (geodesic.h:166); min(u, 1e20) maps inf and NaN there, and any
+        // u >= 1e20 is rejected with fac clamped to 0.2 all the same.
+        const double u = fmin(err2 * 0.125, 1e20);
         // fac = 0.9 err^-alpha err_prev^beta (geodesic.h:170-173) as one
-        // exponential, exp(beta log err_prev - alpha log err)
-        const double lg = 0.5 * log_pos(dmax(u, 1e-16 * 1e-16));
-        double fac = 0.9 * exp_near(lpw - kAlpha * lg);
-        fac = dclamp(fac, 0.2, 5.0);
+        // exponential, exp(beta log err_prev - alpha log err); u, lg, fac are
+        // never NaN here, so max/min are the reference's std::max/std::clamp
+        const double lg = 0.5 * log_pos(fmax(u, 1e-16 * 1e-16));
+        const double fac = fmin(fmax(0.9 * exp_near(lpw - kAlpha * lg), 0.2), 5.0);
         ++natt;
         // Hairer's fifth dense-output coefficient of component c, built only on
         // the steps that locate an event or sample the jet
@@ -581,7 +582,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
             }
         } else {
             // ---- rejected (geodesic.h:192, render.cpp:135) ----------------------
-            h *= dmin(fac, 1.0);
+            h *= fmin(fac, 1.0);
             if (++tries > 40 || fabs(h) < 1e-14) {
                 term = BHX_HORIZON;  // Hit::y keeps the initial photon state
                 const double* I = P.init;
