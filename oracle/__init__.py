@@ -288,3 +288,74 @@ def integrate(a, y, h0, rtol, atol, max_steps, r_min, r_max):
     n = C.c_int64(0)
     rc = lib().oracle_integrate(a, _ptr(y), h0, rtol, atol, max_steps, r_min, r_max, C.byref(n))
     return y, rc, n.value
+
+
+# ---- PNG / APNG writer (src/png.cpp; reference image.cpp:193-778) -----------
+def _encode(fn, *args):
+    L = lib()
+    n = C.c_size_t(0)
+    rc = fn(*args, None, C.c_size_t(0), C.byref(n))
+    if rc < 0:
+        raise ValueError("invalid image")
+    out = np.empty(n.value, np.uint8)
+    rc = fn(*args, out.ctypes.data_as(P), C.c_size_t(n.value), C.byref(n))
+    if rc != 0:
+        raise RuntimeError("encoder failed")
+    return out.tobytes()
+
+
+def png_encode(rgb):
+    """img::write_png's bytes for an (h, w, 3) uint8 image."""
+    rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
+    h, w, _ = rgb.shape
+    L = lib()
+    L.oracle_png_encode.argtypes = [P, C.c_int32, C.c_int32, P, C.c_size_t, C.POINTER(C.c_size_t)]
+    return _encode(L.oracle_png_encode, rgb.ctypes.data_as(P), w, h)
+
+
+def apng_encode(frames, fps, loops=0):
+    """img::write_apng's bytes for an (n, h, w, 3) uint8 stack of frames."""
+    frames = np.ascontiguousarray(frames, dtype=np.uint8)
+    n, h, w, _ = frames.shape
+    L = lib()
+    L.oracle_apng_encode.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P, C.c_size_t,
+                                     C.POINTER(C.c_size_t)]
+    return _encode(L.oracle_apng_encode, frames.ctypes.data_as(P), n, w, h, fps, loops)
+
+
+def huffman_lengths(freq, max_bits):
+    """The writer's Huffman code-length builder (image.cpp:270-334)."""
+    freq = np.ascontiguousarray(freq, dtype=np.uint32)
+    out = np.zeros(len(freq), np.uint8)
+    L = lib()
+    L.oracle_huffman_lengths.argtypes = [P, C.c_int32, C.c_int32, P]
+    L.oracle_huffman_lengths(freq.ctypes.data_as(P), len(freq), max_bits, out.ctypes.data_as(P))
+    return out
+
+
+def _ref_file(call):
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "out.png")
+        if call(path.encode()) != 0:
+            raise RuntimeError("reference writer failed")
+        with open(path, "rb") as f:
+            return f.read()
+
+
+def ref_write_png(rgb):
+    """The reference's own img::write_png (oracle/_ref), as bytes."""
+    rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
+    h, w, _ = rgb.shape
+    R = ref()
+    R.bhref_write_png.argtypes = [C.c_char_p, P, C.c_int32, C.c_int32]
+    return _ref_file(lambda p: R.bhref_write_png(p, rgb.ctypes.data_as(P), w, h))
+
+
+def ref_write_apng(frames, fps, loops=0):
+    """The reference's own img::write_apng (oracle/_ref), as bytes."""
+    frames = np.ascontiguousarray(frames, dtype=np.uint8)
+    n, h, w, _ = frames.shape
+    R = ref()
+    R.bhref_write_apng.argtypes = [C.c_char_p, P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+    return _ref_file(lambda p: R.bhref_write_apng(p, frames.ctypes.data_as(P), n, w, h, fps, loops))

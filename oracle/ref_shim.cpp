@@ -193,4 +193,18 @@ int bhref_polytrope_solve(double K_si, double Gamma, double rho_c, double* R_m, 
     return st.ok ? 0 : 1;
 }
 
+// img::write_png / img::write_apng (image.cpp:652-762) into a file at path.
+int bhref_write_png(const char* path, const uint8_t* rgb, int32_t w, int32_t h) {
+    const std::vector<uint8_t> v(rgb, rgb + static_cast<size_t>(w) * h * 3);
+    return img::write_png(path, v, w, h) ? 0 : 1;
+}
+
+int bhref_write_apng(const char* path, const uint8_t* frames, int32_t n, int32_t w, int32_t h, int32_t fps,
+                     int32_t loops) {
+    const size_t sz = static_cast<size_t>(w) * h * 3;
+    std::vector<std::vector<uint8_t>> f;
+    for (int32_t i = 0; i < n; ++i) f.emplace_back(frames + i * sz, frames + (i + 1) * sz);
+    return img::write_apng(path, f, w, h, fps, loops) ? 0 : 1;
+}
+
 }  // extern "C"

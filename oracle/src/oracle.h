@@ -17,6 +17,7 @@
 #ifndef BH_ORACLE_H
 #define BH_ORACLE_H
 
+#include <stddef.h>
 #include <stdint.h>
 #include "../../include/bhx.h"
 
@@ -92,6 +93,16 @@ double oracle_auto_exposure(const double *xyz, int64_t n_pixels, double key);
 void oracle_add_glare(double *xyz, int32_t w, int32_t h, double strength, double radius_px);
 void oracle_develop(const double *xyz, int64_t n_pixels, double exposure, int tonemap,
                     double log_decades, uint8_t *rgb);
+
+/* img::write_png / img::write_apng (image.cpp:193-778) as byte streams
+ * (src/png.cpp).  rgb: w x h x 3 bytes, row-major; frames: n such images
+ * back to back.  *len gets the stream's size; the bytes are copied to out
+ * when cap >= *len (return 0), else 1 is returned.  -1: invalid arguments. */
+int oracle_png_encode(const uint8_t *rgb, int32_t w, int32_t h, uint8_t *out, size_t cap, size_t *len);
+int oracle_apng_encode(const uint8_t *frames, int32_t n, int32_t w, int32_t h, int32_t fps, int32_t loops,
+                       uint8_t *out, size_t cap, size_t *len);
+/* The writer's Huffman code-length builder (image.cpp:270-334). */
+void oracle_huffman_lengths(const uint32_t *freq, int32_t n, int32_t max_bits, uint8_t *len);
 
 #ifdef __cplusplus
 }
