@@ -84,3 +84,8 @@ class Launch(C.Structure):
 
 
 ORDER_CENTRE, ORDER_COST = 1, 2   # bhx_launch.queue_order
+
+
+class Bytes(C.Structure):
+    """bhx_bytes: a byte stream allocated by the library (bhx_bytes_free)."""
+    _fields_ = [("data", C.POINTER(C.c_uint8)), ("size", C.c_size_t)]

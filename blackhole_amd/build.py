@@ -12,7 +12,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libbhx.so")
 
-HIP_SOURCES = ["tracer.hip", "shade.hip", "api.hip", "order.hip", "batch.hip", "adjoint.hip", "display.hip"]
+HIP_SOURCES = ["tracer.hip", "shade.hip", "api.hip", "order.hip", "batch.hip", "adjoint.hip", "display.hip",
+               "png.hip"]
 # Per-file flags.  The tracer's loop uses ~70 distinct FP64 constants (DP5
 # tableau, sincos/log/exp coefficients); FP64 VALU ops take no 64-bit
 # literals, so machine LICM would pin every one of them in two registers for

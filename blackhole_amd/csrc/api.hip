@@ -157,7 +157,9 @@ int bhx_destroy(bhx_context* ctx) {
                       &ctx->init, &ctx->q2ray, &ctx->evp, &ctx->flux_partial,
                       &ctx->lum_hist, &ctx->lum_npos, &ctx->r_cls, &ctx->r_y, &ctx->r_jet, &ctx->r_xyz,
                       &ctx->pilot_init, &ctx->pilot_q2ray, &ctx->pilot_att, &ctx->pilot_cls, &ctx->order_keys,
-                      &ctx->order_vals, &ctx->order_sizes, &ctx->cost_order, &ctx->cost_start, &ctx->order_tmp})
+                      &ctx->order_vals, &ctx->order_sizes, &ctx->cost_order, &ctx->cost_start, &ctx->order_tmp,
+                      &ctx->png_in, &ctx->png_raw, &ctx->png_key, &ctx->png_pos, &ctx->png_prev, &ctx->png_match,
+                      &ctx->png_tmp, &ctx->png_box, &ctx->png_sub})
         b->release();
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);

@@ -86,6 +86,10 @@ struct bhx_context {
     // and its queue offsets, and the sort/scan workspace
     DevBuf pilot_init, pilot_q2ray, pilot_att, pilot_cls, order_keys, order_vals, order_sizes;
     DevBuf cost_order, cost_start, order_tmp;
+    // PNG / APNG writer: host images uploaded, filtered scanlines, hash keys
+    // and positions (in and out of the sort), chain links, matches (lengths,
+    // distances), sort workspace, changed-pixel box, cropped sub-frame
+    DevBuf png_in, png_raw, png_key, png_pos, png_prev, png_match, png_tmp, png_box, png_sub;
     // initial ray states in queue order (kInitRows x n) and slot -> ray map
     DevBuf init, q2ray;
     // jet / Sun parameters of the tracer's event branches (last uploaded copy)

@@ -6,7 +6,7 @@ module raises, and every entry point that needs the GPU fails loudly.
 import ctypes as C
 import os
 
-from .abi import Launch, Rays, Scene, Shaded, Stats
+from .abi import Bytes, Launch, Rays, Scene, Shaded, Stats
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libbhx.so")
 
@@ -21,7 +21,8 @@ SYMBOLS = (
     "bhx_batch_run", "bhx_batch_scene", "bhx_batch_copy_scene", "bhx_batch_stats",
     "bhx_batch_set_launch", "bhx_batch_get_launch", "bhx_batch_autotune", "bhx_adjoint_create",
     "bhx_adjoint_destroy", "bhx_adjoint_set_scene", "bhx_adjoint_forward", "bhx_adjoint_backward",
-    "bhx_adjoint_rays", "bhx_adjoint_copy_rays", "bhx_adjoint_info",
+    "bhx_adjoint_rays", "bhx_adjoint_copy_rays", "bhx_adjoint_info", "bhx_png_encode", "bhx_png_encode_dev",
+    "bhx_apng_encode", "bhx_apng_encode_dev", "bhx_bytes_free",
 )
 
 
@@ -78,6 +79,14 @@ def lib():
     L.bhx_adjoint_copy_rays.argtypes = [P, P, P, P, P]
     L.bhx_adjoint_info.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                    C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.bhx_png_encode.argtypes = [P, P, C.c_int32, C.c_int32, C.POINTER(Bytes)]
+    L.bhx_png_encode_dev.argtypes = [P, P, C.c_int32, C.c_int32, P, C.POINTER(Bytes)]
+    L.bhx_apng_encode.argtypes = [P, C.POINTER(P), C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                  C.POINTER(Bytes)]
+    L.bhx_apng_encode_dev.argtypes = [P, C.POINTER(P), C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P,
+                                      C.POINTER(Bytes)]
+    L.bhx_bytes_free.argtypes = [C.POINTER(Bytes)]
+    L.bhx_bytes_free.restype = None
     _lib = L
     return L
 

@@ -11,7 +11,7 @@ import ctypes as C
 import numpy as np
 
 from . import native
-from .abi import TM_HDR, Launch, Rays, Scene, Shaded, Stats
+from .abi import TM_HDR, Bytes, Launch, Rays, Scene, Shaded, Stats
 
 
 def _ptr(t):
@@ -174,6 +174,48 @@ class Tracer:
                                                         log_decades, _ptr(rgb), self._stream(stream)),
                      "bhx_develop_dev")
         return rgb.view(-1, 3)
+
+    # ---- PNG / APNG (img::write_png / write_apng, src/image.cpp:193-778) -----
+    def _take(self, out):
+        try:
+            return C.string_at(out.data, out.size)
+        finally:
+            self.lib.bhx_bytes_free(C.byref(out))
+
+    def png_encode(self, rgb, stream=None):
+        """The reference writer's PNG bytes of an (H, W, 3) uint8 image: a
+        numpy array (host) or a contiguous device tensor."""
+        out = Bytes()
+        if isinstance(rgb, np.ndarray):
+            a = np.ascontiguousarray(rgb, dtype=np.uint8)
+            h, w = a.shape[:2]
+            rc = self.lib.bhx_png_encode(self.ctx, a.ctypes.data_as(C.c_void_p), w, h, C.byref(out))
+        else:
+            if rgb.dtype != self.torch.uint8 or rgb.dim() != 3 or not rgb.is_contiguous():
+                raise ValueError("png_encode needs a contiguous (H, W, 3) uint8 tensor")
+            h, w = int(rgb.shape[0]), int(rgb.shape[1])
+            rc = self.lib.bhx_png_encode_dev(self.ctx, _ptr(rgb), w, h, self._stream(stream), C.byref(out))
+        native.check(self.ctx, rc, "bhx_png_encode")
+        return self._take(out)
+
+    def apng_encode(self, frames, fps, loops=0, stream=None):
+        """The reference writer's APNG bytes of a sequence of (H, W, 3) uint8
+        frames: numpy arrays (host) or contiguous device tensors."""
+        out = Bytes()
+        n = len(frames)
+        if n == 0:
+            raise ValueError("apng_encode needs at least one frame")
+        h, w = int(frames[0].shape[0]), int(frames[0].shape[1])
+        if isinstance(frames[0], np.ndarray):
+            keep = [np.ascontiguousarray(f, dtype=np.uint8) for f in frames]
+            ptrs = (C.c_void_p * n)(*[f.ctypes.data for f in keep])
+            rc = self.lib.bhx_apng_encode(self.ctx, ptrs, n, w, h, fps, loops, C.byref(out))
+        else:
+            ptrs = (C.c_void_p * n)(*[_ptr(f) for f in frames])
+            rc = self.lib.bhx_apng_encode_dev(self.ctx, ptrs, n, w, h, fps, loops, self._stream(stream),
+                                              C.byref(out))
+        native.check(self.ctx, rc, "bhx_apng_encode")
+        return self._take(out)
 
     def blackbody_xyz(self, T, stream=None):
         t = self.torch

@@ -30,6 +30,7 @@
 #ifndef BHX_H
 #define BHX_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -357,6 +358,39 @@ int bhx_polytrope_star(double K_si, double Gamma, double M_kg, double *R_m, doub
  * 503-512) and NeutronStar::set_from / set_spin (src/neutronstar.h:69-79);
  * ns_spin_hz must already be set.  Camera framing stays with the caller. */
 int bhx_setup_neutron_star(bhx_scene *scene, double R_m, double M_kg);
+
+/* ---- PNG / APNG ----------------------------------------------------------
+ *
+ * img::write_png(path, rgb, w, h) and img::write_apng(path, frames, w, h, fps,
+ * loops) (src/image.h:45-50, src/image.cpp:193-778) as byte streams: the same
+ * bytes as the reference's writer - per-scanline filter of minimum residual,
+ * one dynamic-Huffman DEFLATE block from the greedy hash-chain LZ77 parse
+ * (stored block when that is smaller), zlib, PNG / APNG 1.0 chunks with the
+ * changed rectangle of each later frame.  The scanline filters and the match
+ * search run on the device; the parse, the Huffman codes and the checksums
+ * on the host.  Images are w x h x 3 bytes, row-major RGB.  The calls return
+ * when the stream is complete (they synchronise `stream`).  The stream is
+ * allocated by the library: release it with bhx_bytes_free.  Unlike the
+ * reference (which writes a file and returns false on a size mismatch) the
+ * sizes are arguments; invalid ones give BHX_E_INVALID.
+ */
+typedef struct bhx_bytes {
+    uint8_t *data;
+    size_t size;
+} bhx_bytes;
+
+/* rgb in host memory */
+int bhx_png_encode(bhx_context *ctx, const uint8_t *rgb, int32_t width, int32_t height, bhx_bytes *out);
+/* rgb in device memory (e.g. bhx_develop_dev's output), ordered on stream */
+int bhx_png_encode_dev(bhx_context *ctx, const uint8_t *rgb, int32_t width, int32_t height, void *stream,
+                       bhx_bytes *out);
+/* n_frames images in host memory; loops = 0 plays forever */
+int bhx_apng_encode(bhx_context *ctx, const uint8_t *const *frames, int32_t n_frames, int32_t width,
+                    int32_t height, int32_t fps, int32_t loops, bhx_bytes *out);
+/* n_frames device pointers (the array itself in host memory), ordered on stream */
+int bhx_apng_encode_dev(bhx_context *ctx, const uint8_t *const *frames, int32_t n_frames, int32_t width,
+                        int32_t height, int32_t fps, int32_t loops, void *stream, bhx_bytes *out);
+void bhx_bytes_free(bhx_bytes *bytes);
 
 #ifdef __cplusplus
 }

@@ -39,7 +39,7 @@ def test_no_gpu_call_needed_for_errors():
 
 @pytest.mark.parametrize("struct,cname", [(abi.Scene, "bhx_scene"), (abi.Stats, "bhx_stats"),
                                           (abi.Rays, "bhx_rays"), (abi.Shaded, "bhx_shaded"),
-                                          (abi.Launch, "bhx_launch")])
+                                          (abi.Launch, "bhx_launch"), (abi.Bytes, "bhx_bytes")])
 def test_struct_layout_matches_header(tmp_path, struct, cname):
     fields = [f[0] for f in struct._fields_]
     src = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{HEADER}"', "int main(void){",
