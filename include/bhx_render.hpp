@@ -17,6 +17,10 @@
  *     }
  *     }
  *
+ * The same header gives img::write_png / img::write_apng (src/image.h:45-50)
+ * on the GPU-assisted writer as bhx::write_png / bhx::write_apng
+ * (blackhole_amd/integration/png_hip.cpp routes the reference's to them).
+ *
  * Errors: the reference's render() has no failure mode; a HIP failure here
  * (no device, out of memory, a scene outside the documented limits) throws
  * std::runtime_error with the library's message.
@@ -29,12 +33,15 @@
 #ifndef BHX_RENDER_HPP
 #define BHX_RENDER_HPP
 
+#include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <vector>
 
 #include "bhx.h"
 
@@ -170,6 +177,47 @@ Image render(const Cfg& cfg, Stats& stats) {
         throw std::runtime_error(std::string("bhx_render failed: ") + bhx_last_error(ctx));
     merge_stats(stats, st);
     return im;
+}
+
+namespace detail {
+inline bool write_bytes(const std::string& path, bhx_bytes& b) {
+    FILE* f = std::fopen(path.c_str(), "wb");
+    bool ok = f != nullptr;
+    if (f) {
+        ok = std::fwrite(b.data, 1, b.size, f) == b.size;
+        std::fclose(f);
+    }
+    bhx_bytes_free(&b);
+    return ok;
+}
+}  // namespace detail
+
+/* img::write_png / img::write_apng (src/image.h:45-50) on the GPU-assisted
+ * writer: the same file bytes as the reference's (src/image.cpp:193-778).
+ * false on a size mismatch or when the file cannot be written, as the
+ * reference; a HIP failure throws std::runtime_error. */
+inline bool write_png(const std::string& path, const std::vector<uint8_t>& rgb, int w, int h) {
+    if (w <= 0 || h <= 0 || static_cast<size_t>(w) * h * 3 != rgb.size()) return false;
+    bhx_context* ctx = thread_context();
+    bhx_bytes b{nullptr, 0};
+    if (bhx_png_encode(ctx, rgb.data(), w, h, &b) != BHX_OK)
+        throw std::runtime_error(std::string("bhx_png_encode failed: ") + bhx_last_error(ctx));
+    return detail::write_bytes(path, b);
+}
+
+inline bool write_apng(const std::string& path, const std::vector<std::vector<uint8_t>>& frames, int w, int h,
+                       int fps, int loops) {
+    if (frames.empty() || w <= 0 || h <= 0) return false;
+    std::vector<const uint8_t*> ptr;
+    for (const auto& f : frames) {
+        if (static_cast<size_t>(w) * h * 3 != f.size()) return false;
+        ptr.push_back(f.data());
+    }
+    bhx_context* ctx = thread_context();
+    bhx_bytes b{nullptr, 0};
+    if (bhx_apng_encode(ctx, ptr.data(), static_cast<int32_t>(ptr.size()), w, h, fps, loops, &b) != BHX_OK)
+        throw std::runtime_error(std::string("bhx_apng_encode failed: ") + bhx_last_error(ctx));
+    return detail::write_bytes(path, b);
 }
 
 }  // namespace bhx

@@ -1,14 +1,21 @@
 """The drop-in end to end: the reference's own program built twice from its
-sources (oracle/Makefile target `cli`), once as shipped (CPU render()) and
-once with src/render.cpp replaced by blackhole_amd/integration/render_hip.cpp
-over libbhx.so.  Both run the same command lines; their PNGs must agree within
-1 LSB per channel, apart from a few pixels whose samples sit on a class
-boundary.  Needs an MI355X; skipped where the two programs were not built."""
+sources (oracle/Makefile target `cli`), once as shipped (CPU render() and PNG
+writer) and once with src/render.cpp replaced by
+blackhole_amd/integration/render_hip.cpp and img::write_png / write_apng
+routed to blackhole_amd/integration/png_hip.cpp, over libbhx.so.  Both run the
+same command lines; their images must agree within 1 LSB per channel, apart
+from a few pixels whose samples sit on a class boundary, and the GPU program's
+files must be the reference writer's bytes for their pixels (the oracle's
+writer, pinned to the reference's) - byte-identical to the CPU program's
+files wherever the pixels are.  Needs an MI355X; skipped where the two
+programs were not built."""
 import os
 import subprocess
 
 import numpy as np
 import pytest
+
+import oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -24,6 +31,7 @@ CASES = {
     "sun": ["--sun"],
     "sgra": ["--preset", "sgra"],
     "hotspot_frames": ["--hotspot", "8", "--frames", "3"],
+    "hotspot_apng": ["--hotspot", "8", "--frames", "4", "--fps", "12"],
 }
 
 
@@ -31,6 +39,13 @@ def _png(path):
     from PIL import Image
     with Image.open(path) as im:
         return np.asarray(im.convert("RGB"), dtype=np.int16)
+
+
+def _apng_frames(path):
+    """The composed frames of an APNG (Pillow's APNG reader)."""
+    from PIL import Image, ImageSequence
+    with Image.open(path) as im:
+        return [np.asarray(f.convert("RGB"), dtype=np.int16) for f in ImageSequence.Iterator(im)]
 
 
 def _run(binary, args, out):
@@ -46,6 +61,19 @@ def _run(binary, args, out):
 def test_reference_program_with_gpu_render(tmp_path, name):
     args = CASES[name]
     print(f"[dropin {name}] {' '.join(args)}", flush=True)
+    if name.endswith("apng"):
+        _run(CPU_BIN, args, str(tmp_path / "cpu.apng"))
+        _run(HIP_BIN, args, str(tmp_path / "hip.apng"))
+        a, b = (tmp_path / "cpu.apng").read_bytes(), (tmp_path / "hip.apng").read_bytes()
+        fa, fb = _apng_frames(tmp_path / "cpu.apng"), _apng_frames(tmp_path / "hip.apng")
+        assert len(fa) == len(fb) == 4
+        for x, y in zip(fa, fb):
+            assert int((np.abs(x - y).max(axis=2) > 1).sum()) <= max(2, x.size // 3000)
+        frames = np.stack([f.astype(np.uint8) for f in fb])
+        assert b == oracle.apng_encode(frames, 12, 0)       # the reference writer's bytes
+        if all(np.array_equal(x, y) for x, y in zip(fa, fb)):
+            assert a == b
+        return
     _run(CPU_BIN, args, str(tmp_path / "cpu.png"))
     _run(HIP_BIN, args, str(tmp_path / "hip.png"))
     if "--frames" in args:
@@ -61,3 +89,7 @@ def test_reference_program_with_gpu_render(tmp_path, name):
         print(f"[dropin {name}] {os.path.basename(b)}: max diff {int(d.max())} LSB, "
               f"{bad} px above 1 LSB, mean |diff| {np.abs(ca - hb).mean():.4f}", flush=True)
         assert bad <= max(2, d.size // 1000), (name, bad)
+        # the GPU program's file is the reference writer's bytes for its pixels
+        assert b.read_bytes() == oracle.png_encode(hb.astype(np.uint8)), name
+        if np.array_equal(ca, hb):
+            assert a.read_bytes() == b.read_bytes(), name
