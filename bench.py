@@ -321,6 +321,38 @@ def sweep_leg(tr, torch, n_frames=120, width=1024, K=64, rounds=2, grad=True, fd
     return res
 
 
+def png_leg(tr, torch, sc, xyz, rounds=5, cpu=True):
+    """The frame's PNG (img::write_png, src/image.cpp:193-778) from device
+    memory: auto exposure + develop on the device, then the GPU-assisted
+    writer (scanline filters and the LZ77 match search on the device, parse
+    and Huffman coding on the host).  Checked byte for byte against the
+    oracle's restatement of the reference's writer, which is also timed as
+    the single-threaded CPU baseline (the reference's writer is one thread)."""
+    e = tr.auto_exposure(xyz)
+    rgb = tr.develop(xyz, e).view(sc.height, sc.width, 3)
+    b = tr.png_encode(rgb)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(rounds):
+        b = tr.png_encode(rgb)
+    ms = (time.perf_counter() - t0) / rounds * 1e3
+    res = {"workload": f"img::write_png of the developed {sc.width}x{sc.height} frame "
+                       f"({3 * sc.width * sc.height} bytes of sRGB in device memory)",
+           "png_bytes": len(b), "gpu_assisted_ms": round(ms, 3),
+           "mb_per_s": round(3 * sc.width * sc.height / (ms * 1e-3) / 1e6, 1)}
+    if cpu:
+        import oracle
+        host = rgb.cpu().numpy()
+        t0 = time.perf_counter()
+        ref = oracle.png_encode(host)
+        cms = (time.perf_counter() - t0) * 1e3
+        res["identical_to_oracle"] = ref == b
+        res["cpu_baseline"] = {"value": round(cms, 3), "unit": "ms", "cores": 1, "kind": "port",
+                               "sample": "the same frame through oracle/ (the reference writer restated, "
+                                         "byte-identical to it), one thread like the reference"}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -332,6 +364,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation leg")
     ap.add_argument("--no-fd", action="store_true", help="skip the finite-difference check of the gradient")
     ap.add_argument("--no-sweep", action="store_true", help="skip the quasar-sweep leg")
+    ap.add_argument("--no-png", action="store_true", help="skip the PNG-writer leg")
     ap.add_argument("--sweep-frames", type=int, default=120)
     ap.add_argument("--sweep-width", type=int, default=1024)
     ap.add_argument("--no-sweep-grad", action="store_true", help="sweep leg without the per-frame gradients")
@@ -430,6 +463,8 @@ def main():
                              "peak, equal to its FP64 matrix peak",
                      "flops_per_launch": alg_flops},
     }
+    if not args.no_png:
+        line["png"] = png_leg(tr, torch, sc, out["xyz"], cpu=not args.no_cpu)
     if not args.no_batch:
         line["batched"] = batched_leg(tr, torch)
         line["batched_over_plain"] = line["batched"]["batched_over_plain"]

@@ -110,28 +110,36 @@ __global__ void __launch_bounds__(256) k_png_links(const uint16_t* __restrict__ 
 }
 
 // the longest match at i among the chain's first 192 candidates within the
-// window; the first one found wins ties; a full-length match stops the walk
+// window; the first one found wins ties; a full-length match stops the walk.
+// A candidate can only be longer than the best so far if it agrees at the
+// best's length, so that one byte is tested before the full comparison (the
+// result is the same: a candidate that fails it is at most as long).
 __global__ void __launch_bounds__(256) k_png_match(const uint8_t* __restrict__ d, int64_t n,
                                                    const int32_t* __restrict__ prev, uint16_t* __restrict__ mlen,
                                                    uint16_t* __restrict__ mdist) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i + 2 >= n) return;
     const int cap = static_cast<int>(min<int64_t>(kMaxMatch, n - i));
+    const uint8_t* q = d + i;
     int best = 0, bdist = 0;
+    uint8_t qb = q[0];   // q[best]
     int cand = prev[i];
     for (int chain = 0; cand >= 0 && chain < kMaxChain; ++chain) {
         const int64_t dist = i - cand;
         if (dist > kWin) break;
         const uint8_t* p = d + cand;
-        const uint8_t* q = d + i;
-        int l = 0;
-        while (l < cap && p[l] == q[l]) ++l;
-        if (l > best) {
-            best = l;
-            bdist = static_cast<int>(dist);
-            if (best >= cap) break;
+        const int next = prev[cand];
+        if (p[best] == qb) {
+            int l = 0;
+            while (l < cap && p[l] == q[l]) ++l;
+            if (l > best) {
+                best = l;
+                bdist = static_cast<int>(dist);
+                if (best >= cap) break;
+                qb = q[best];
+            }
         }
-        cand = prev[cand];
+        cand = next;
     }
     mlen[i] = static_cast<uint16_t>(best);
     mdist[i] = static_cast<uint16_t>(bdist);   // <= 32768
