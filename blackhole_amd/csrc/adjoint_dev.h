@@ -111,7 +111,7 @@ __host__ __device__ inline void attempt(const RayK& K, const Ray& s, double rtol
     for (int c = 0; c < 6; ++c) {
         const double e = h * (E1 * k1[c] + E3 * k3[c] + E4 * k4[c] + E5 * k5[c] + E6 * k6[c] + E7 * o.g7.d[c]);
         const double sc = atol + rtol * fmax(fabs(y[c]), fabs(o.yn[c]));   // y is never NaN
-        const double qe = e * rcp(sc);
+        const double qe = e * rcp_coarse(sc);
         err2 += qe * qe;
         o.c5[c] = h * (D1 * k1[c] + D3 * k3[c] + D4 * k4[c] + D5 * k5[c] + D6 * k6[c] + D7 * o.g7.d[c]);
     }

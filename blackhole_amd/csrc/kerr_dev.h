@@ -34,6 +34,18 @@ __host__ __device__ __forceinline__ double rcp(double x) {
 #endif
 }
 
+// 1/x to 2.2e-15 relative (hardware estimate, 2^-24.4, + one Newton step;
+// measured over 4M arguments): for the error norm's scales, a step-size
+// heuristic that needs no more.
+__host__ __device__ __forceinline__ double rcp_coarse(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
+#else
+    return 1.0 / x;
+#endif
+}
+
 // sin and cos of |x| <= 1e5 to <= 1 ulp (checked against glibc): Cody-Waite
 // reduction by pi/2 with a two-term split evaluated by FMA, then the minimax
 // kernels of Sun's fdlibm (__kernel_sin / __kernel_cos, 1993) on

@@ -376,7 +376,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
                                   E7 * g7.d[c]);
             // max of two magnitudes: y is an accepted state, so never NaN
             const double sc = latol + lrtol * fmax(fabs(y[c]), fabs(yn[c]));
-            const double qe = e * rcp(sc);
+            const double qe = e * rcp_coarse(sc);
             err2 += qe * qe;   // the p_t, p_phi terms are exactly 0
         }
         // The RMS error is err = sqrt(u), u = err2 / 8 (geodesic.h:157-166),
