@@ -118,7 +118,12 @@ __device__ void sky_radiance(const ShadeArgs& P, const double d[3], double o[3])
     o[2] += amp * P.mw_cz;
 }
 
-__global__ void __launch_bounds__(256) k_shade(const ShadeArgs P) {
+// Two waves per SIMD: unconstrained, the class branches (library acos, atan2,
+// pow, trig) take ~300 registers and one wave per SIMD, which leaves the
+// star-cell gathers' latency bare; capped at 256 the few spills fall in the
+// rare branches (0.29 -> 0.22 ms per 1024^2 frame; 3 waves spill in the
+// common ones and are slower).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_shade(const ShadeArgs P) {
     __shared__ unsigned long long counts[7];
     __shared__ double flux_w[4];
     __shared__ unsigned long long ttot[4];
