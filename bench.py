@@ -78,13 +78,46 @@ def cpu_baseline(sc, threads):
     dt = time.perf_counter() - t0
     steps = int(out["steps"].sum())
     att = int(out["attempts"].sum())
+    flops = ALG_FLOPS_PER_ACCEPTED_STEP * steps + ALG_FLOPS_PER_REJECTED_ATTEMPT * (att - steps)
     return {
         "value": len(ids) / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-        "sample": f"the whole 1024x1024 frame ({len(ids)} rays, {steps} accepted steps)"
+        "sample": f"the whole {sc.width}x{sc.height} frame ({len(ids)} rays, {steps} accepted steps)"
                   f" traced+shaded by oracle/ (bitwise-equal CPU restatement) in {dt:.2f} s",
         "rhs_evals_per_s": (6 * att + len(ids)) / dt,
-        "cpu_model": cpu_model(),
+        "accepted_steps": steps, "rejected_attempts": att - steps,
+        "alg_gflops": round(flops / dt / 1e9, 2),
+        "nproc": os.cpu_count(), "cpu_model": cpu_model(),
     }
+
+
+def small_frame_leg(tr, torch, threads, rounds=20, cpu=True):
+    """The reference's CPU-runnable frame: 256x256, a = 0.9, camera at 500 M
+    and 80 deg, +-20 M image plane, NT disc ISCO-20 M at 0.1 Eddington, no
+    stars, rtol 1e-10 / atol 1e-12.  256^2 rays are fewer than the resident
+    lanes, so the GPU frame is one ray lifetime long; reported beside the
+    headline, not as it."""
+    from blackhole_amd import scene as S
+    sc = S.plain_frame()
+    rays = tr.alloc_rays(sc, full=False)
+    out = {"xyz": torch.empty((sc.height, sc.width, 3), dtype=torch.float64, device="cuda:0")}
+    for _ in range(3):
+        tr.trace_async(sc, rays)
+        tr.shade_async(sc, rays, out)
+    _, st = tr.trace(sc, rays)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(rounds):
+        tr.trace_async(sc, rays)
+        tr.shade_async(sc, rays, out)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / rounds * 1e3
+    res = {"workload": "plain Kerr frame 256x256, a=0.9, i=80 deg, r_cam=500M, +-20M, NT disc ISCO-20M "
+                       "@0.1 Edd, no stars, rtol 1e-10 / atol 1e-12, 1 spp",
+           "ms_per_frame": round(ms, 4), "mrays_s": round(sc.n_rays / (ms * 1e-3) / 1e6, 3),
+           "accepted_steps_per_ray": st.steps / sc.n_rays}
+    if cpu:
+        res["cpu_baseline"] = cpu_baseline(sc, threads)
+    return res
 
 
 def batched_leg(tr, torch, rounds=3):
@@ -465,6 +498,7 @@ def main():
     }
     if not args.no_png:
         line["png"] = png_leg(tr, torch, sc, out["xyz"], cpu=not args.no_cpu)
+    line["small_frame"] = small_frame_leg(tr, torch, min(16, os.cpu_count() or 1), cpu=not args.no_cpu)
     if not args.no_batch:
         line["batched"] = batched_leg(tr, torch)
         line["batched_over_plain"] = line["batched"]["batched_over_plain"]
