@@ -3,7 +3,7 @@
 One step = one pass of the hot path over one frame: the persistent tracer over
 all 1,048,576 rays plus the shading pass (disc emission, star-cell lookup,
 colour), inputs (scene, star catalogue, colour table) already resident on the
-GPU.  Prints one JSON line; see DESIGN.md for the definitions of every field.
+GPU.  Prints one JSON line (the fields are defined where they are computed below).
 """
 import argparse
 import json
@@ -405,8 +405,8 @@ def main():
                     help="run a 256 MiB device copy first (for HBM-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only; per-GPU sharding is described in "
-              "DESIGN.md (no multi-GPU figure is measured)", file=sys.stderr)
+        print("bench.py measures one GPU only (--gpus 1): the path shards by image tiles or by "
+              "frames across GPUs, and no multi-GPU figure is measured", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -109,7 +109,7 @@ def setup_jet(s, phi_flux=50.0, eps_rad=0.02):
 
 def setup_conical_jet(s, half_angle_deg=5.0, r0=6.0, r1=500.0, gamma=10.0, alpha=0.7,
                       phi_flux=50.0, eps_rad=0.02):
-    """Give the scene the build's conical jet (no reference counterpart): two
+    """Give the scene a conical jet model of this package (no reference counterpart): two
     cones of half opening half_angle_deg about the spin axis from r0 to r1
     (GM/c^2), radial outflow at Lorentz factor gamma, comoving emissivity
     ~ (r0/r)^2 nu^-alpha, normalised like the reference's jets (the radiated

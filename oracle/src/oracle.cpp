@@ -666,7 +666,7 @@ struct Disc {  // disc.h:66-151
 struct JetModel {
     double z_base, z_top, R_base, k, gamma_max, z_sat, alpha, u_peak, u_wid, spine, spine_wid;
     double zeta, Omega_F, j_nu0, r_bound;
-    // conical jet (bhx_scene.jet_model = 1), the build's own model: no
+    // conical jet (bhx_scene.jet_model = 1), this package's own model: no
     // reference counterpart; the same formulas as the device's cone_shape /
     // cone_boost, written with the C library's functions.  Profile (r0/r)^2
     // x taper x end ramps, all three with the quintic step (the taper in
@@ -941,7 +941,7 @@ void trace_state(const Scene& sc, const double y0[8], RayResult& out) {
     double p0[3];
     to_cartesian(a, y[YR], y[YH], y[YP], p0);
 
-    // the conical jet (the build's own model) has a sampler of its own so
+    // the conical jet (this package's own model) has a sampler of its own so
     // that the reference's path stays the reference's code
     auto cone_sample = [&](const Seg& seg, double h_step, double s_end) {
         const int nsub = 8;

@@ -1,4 +1,4 @@
-"""Host normalisation of the conical jet (bhx_setup_conical_jet, the build's
+"""Host normalisation of the conical jet (bhx_setup_conical_jet, the package's
 own jet model: no reference counterpart) and the band factor that turns its
 specific intensity into intensity.  CPU only.
 

@@ -1,4 +1,4 @@
-"""The quasar sweep's content on the MI355X: the build's conical jet (no
+"""The quasar sweep's content on the MI355X: the package's conical jet (no
 reference counterpart: checked against the oracle's restatement of the same
 definition), a 1000 M disc and the star field, and a sweep of frames as one
 batched launch.  Needs an MI355X."""
