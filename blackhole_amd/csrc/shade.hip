@@ -25,8 +25,9 @@ constexpr double kSigmaSB = 5.670374419e-8;
 __device__ __forceinline__ void bb_lookup(const double* __restrict__ tab, double T, double o[3]) {
     const double lg = log10(dclamp(T, 50.0, 3.0e9));
     const double f = (lg - kLgMin) / (kLgMax - kLgMin) * (kTabN - 1);
-    int i = (int)f;
-    i = i < 0 ? 0 : (i > kTabN - 2 ? kTabN - 2 : i);
+    // A NaN temperature (clamp keeps it) gives row 0 and u = NaN: a NaN colour,
+    // as the reference's; the conversion is guarded so NaN never reaches (int).
+    const int i = (f > 0.0) ? (int)dmin(f, (double)(kTabN - 2)) : 0;
     const double u = f - i;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -57,10 +58,15 @@ __device__ double disc_T(const ShadeArgs& P, double r) {
     const double root = sqrt(r * r - 3.0 * r + 2.0 * sg * a * sr);
     const double E = (r * r - 2.0 * r + sg * a * sr) / (r * root);
     const double L = (sg * sr * (r * r - 2.0 * sg * a * sr + a * a)) / (r * root);
+    // flux_dimensionless's rejections give 0, which flux() still multiplies by
+    // the scale: 0 * NaN is NaN when the efficiency is NaN (a < 0 with a
+    // prograde disc), and temperature() then rejects only F <= 0.
     const double den = E - Om * L;
-    if (!(den > 0.0)) return 0.0;
-    double F = (-dOm) / (4.0 * kPi * r * den * den) * pt_W(P, r);
-    if (!(F > 0.0 && isfinite(F))) return 0.0;
+    double F = 0.0;
+    if (den > 0.0) {
+        F = (-dOm) / (4.0 * kPi * r * den * den) * pt_W(P, r);
+        if (!(F > 0.0 && isfinite(F))) F = 0.0;
+    }
     F = F * P.flux_scale;
     if (F <= 0.0) return 0.0;
     return pow(F / kSigmaSB, 0.25);
@@ -158,7 +164,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
             if (cls == BHX_DISC && P.has_disc) {
                 const double r = y[1];
                 double T = disc_T(P, r);
-                if (T > 0.0) {
+                // render.cpp:278 rejects T <= 0 only: a NaN temperature (a < 0
+                // with a prograde disc makes the efficiency NaN) goes on to g
+                // and a NaN colour there, and does here.
+                if (!(T <= 0.0)) {
                     if (P.hs_enabled) {
                         const double t_emit = P.observer_time + y[0];
                         const double phs = P.hs_phi0 + P.hs_omega * t_emit;
