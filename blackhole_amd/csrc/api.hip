@@ -21,7 +21,9 @@ int check_scene(bhx_context* ctx, const bhx_scene* s) {
     const int S = std::max(1, s->sqrt_spp);
     const double n = static_cast<double>(s->width) * s->height * S * S;
     if (n > 2.0e9) return fail(ctx, BHX_E_TOO_LARGE, "more than 2e9 samples in one frame");
-    if (!(s->rtol > 0.0) || !(s->atol > 0.0)) return fail(ctx, BHX_E_INVALID, "rtol/atol must be > 0");
+    // rtol / atol are taken as given, as Stepper does (render.cpp:80): rtol = 0 is a
+    // pure absolute tolerance; a scale of 0 makes the error NaN, the attempt is
+    // rejected, and after 40 retries the ray ends as captured.
     return BHX_OK;
 }
 

@@ -117,7 +117,6 @@ int bhx_batch_create(bhx_context* ctx, const bhx_scene* scenes, int32_t n, bhx_b
     for (int i = 0; i < n; ++i) {
         const bhx_scene& s = scenes[i];
         if (s.width <= 0 || s.height <= 0) return bhx_fail(ctx, BHX_E_INVALID, "width/height must be > 0");
-        if (!(s.rtol > 0.0) || !(s.atol > 0.0)) return bhx_fail(ctx, BHX_E_INVALID, "rtol/atol must be > 0");
         const int S = std::max(1, s.sqrt_spp);
         total += static_cast<int64_t>(s.width) * s.height * S * S;
         total_pix += static_cast<int64_t>(s.width) * s.height;
