@@ -33,6 +33,7 @@
 #ifndef BHX_RENDER_HPP
 #define BHX_RENDER_HPP
 
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -165,7 +166,14 @@ inline bhx_context* thread_context() {
  * doubles per pixel, row-major (src/image.h:18-27). */
 template <class Image, class Cfg, class Stats>
 Image render(const Cfg& cfg, Stats& stats) {
+    const auto t0 = std::chrono::steady_clock::now();
     Image im(cfg.width, cfg.height);
+    if (cfg.width == 0 || cfg.height == 0) {
+        // no samples: the reference's loops do nothing and it returns the empty
+        // image with only the time assigned (src/render.cpp:396-497)
+        stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return im;
+    }
     using Px = typename std::decay<decltype(im.pixels[0])>::type;
     static_assert(sizeof(Px) == 3 * sizeof(double), "pixel must be three doubles");
     const bhx_scene s = scene_of(cfg);
