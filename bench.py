@@ -398,6 +398,7 @@ def main():
     ap.add_argument("--no-fd", action="store_true", help="skip the finite-difference check of the gradient")
     ap.add_argument("--no-sweep", action="store_true", help="skip the quasar-sweep leg")
     ap.add_argument("--no-png", action="store_true", help="skip the PNG-writer leg")
+    ap.add_argument("--no-small-frame", action="store_true", help="skip the 256x256 plain-frame leg")
     ap.add_argument("--sweep-frames", type=int, default=120)
     ap.add_argument("--sweep-width", type=int, default=1024)
     ap.add_argument("--no-sweep-grad", action="store_true", help="sweep leg without the per-frame gradients")
@@ -498,7 +499,8 @@ def main():
     }
     if not args.no_png:
         line["png"] = png_leg(tr, torch, sc, out["xyz"], cpu=not args.no_cpu)
-    line["small_frame"] = small_frame_leg(tr, torch, min(16, os.cpu_count() or 1), cpu=not args.no_cpu)
+    if not args.no_small_frame:
+        line["small_frame"] = small_frame_leg(tr, torch, min(16, os.cpu_count() or 1), cpu=not args.no_cpu)
     if not args.no_batch:
         line["batched"] = batched_leg(tr, torch)
         line["batched_over_plain"] = line["batched"]["batched_over_plain"]

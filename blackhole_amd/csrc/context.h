@@ -61,7 +61,7 @@ struct bhx_context {
     int device = 0;
     int n_cu = 0;
     std::string err;
-    bhx_launch launch{256, 0, 32, 8, BHX_ORDER_COST};   // chunk 32, refill 8: fastest of a sweep on the 1024² frame
+    bhx_launch launch{256, 0, 32, 1, BHX_ORDER_COST};   // chunk 32, refill 1: fastest of a sweep on the 1024² frame
     // tables
     double* d_bb = nullptr;
     // catalogue (cell CSR + star rows)
