@@ -158,6 +158,14 @@ void bhref_blackbody_xyz(double T, double* o) {
     o[0] = c.x; o[1] = c.y; o[2] = c.z;
 }
 
+double bhref_blackbody_bolometric(double T) { return spec::blackbody_bolometric(T); }
+
+// NovikovThorneDisc::temperature (disc.h:100-104) of the scene's disc at r.
+double bhref_disc_temperature(const bhx_scene* s, double r) {
+    const bh::NovikovThorneDisc d(s->M_kg, s->spin, s->disc_r_out, s->eddington, s->disc_sense);
+    return d.temperature(r);
+}
+
 void bhref_star_radiance(int32_t count, uint64_t seed, const double* d, double sigma, double* o) {
     const bh::StarField sf(count, seed);
     const spec::XYZ c = sf.radiance(d, sigma);

@@ -1530,22 +1530,57 @@ double oracle_disc_temperature(const bhx_scene* s, double r) {
     const Disc d(s->M_kg, s->spin, s->disc_r_out, s->eddington, s->disc_sense);
     return d.temperature(r);
 }
-// Bolometric intensity sigma (g T)^4 / pi of the disc hits among n rays
-// (class 1 with g > 0 and T > 0), else 0: the loss input of the
-// differentiable disc image.
+// Bolometric intensity of the disc hits among n rays (class 1 with g > 0 and
+// T > 0), else 0: spec::blackbody_bolometric (spectrum.cpp:24-26) of the
+// observed temperature g T, sigma T^4 / pi in the reference's operation
+// order; the loss input of the differentiable disc image.
 void oracle_disc_intensity(const bhx_scene* s, int64_t n, const uint8_t* cls, const double* r, const double* g,
                            double* out) {
     const Disc d(s->M_kg, s->spin, s->disc_r_out, s->eddington, s->disc_sense);
-    const double sigma = 5.670374419e-8, pi = 3.14159265358979323846;
     for (int64_t i = 0; i < n; ++i) {
         out[i] = 0.0;
         if (cls[i] != 1 || !(g[i] > 0.0)) continue;
         const double T = d.temperature(r[i]);
         if (!(T > 0.0)) continue;
         const double gT = g[i] * T;
-        out[i] = sigma * (gT * gT) * (gT * gT) / pi;
+        out[i] = kSigmaSB * gT * gT * gT * gT / kPi;
     }
 }
+// ---- conical jet pieces (this package's own jet model), for the checks ----
+// Four-velocity of the jet's radial outflow at (a, r, theta) for a Lorentz
+// factor gamma relative to the zero-angular-momentum observer.
+void oracle_cone_velocity(double a, double r, double th, double gamma, double* u) {
+    JetModel j{};
+    j.gamma_max = gamma;
+    j.cone_velocity(point(a, r, th), u);
+}
+// Emissivity profile of the scene's conical jet at (r, cos theta), n points.
+void oracle_cone_shape(const bhx_scene* s, int64_t n, const double* r, const double* cs, double* out) {
+    JetModel j{};
+    j.model = 1;
+    j.cone_angle = s->jet_cone_angle;
+    j.cone_r0 = s->jet_cone_r0;
+    j.cone_r1 = s->jet_cone_r1;
+    for (int64_t i = 0; i < n; ++i) out[i] = j.cone_shape(r[i], cs[i]);
+}
+// The sampling gate of the conical jet for n steps whose dense-output theta
+// coefficients are c[k * n + i], k = 0..4 (c1..c5 of the Hairer interpolant).
+void oracle_cone_gate(const bhx_scene* s, int64_t n, const double* c, uint8_t* out) {
+    JetModel j{};
+    j.model = 1;
+    j.cone_angle = s->jet_cone_angle;
+    for (int64_t i = 0; i < n; ++i)
+        out[i] = j.cone_gate(c[i], c[n + i], c[2 * n + i], c[3 * n + i], c[4 * n + i]) ? 1 : 0;
+}
+// The redshift factor g = 1 / (p . u) of a photon state for an emitter of
+// four-velocity u (render.cpp:31-34), and the jet's boost g^(2 + alpha).
+double oracle_redshift(const double* y, const double* u) { return redshift(y, u); }
+double oracle_jet_boost(double alpha, double g) {
+    JetModel j{};
+    j.alpha = alpha;
+    return j.boost(g);
+}
+
 void oracle_blackbody_xyz(double T, double* o) { bb_xyz(T, o); }
 void oracle_blackbody_xyz_exact(double T, double* o) { bb_exact(T, o); }
 void oracle_powerlaw_xyz(double alpha, double* o) { powerlaw(alpha, o); }

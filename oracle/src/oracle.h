@@ -80,6 +80,12 @@ double oracle_disc_temperature(const bhx_scene *sc, double r);
 void oracle_flop_counts(int64_t *out);
 void oracle_disc_intensity(const bhx_scene *sc, int64_t n, const uint8_t *cls, const double *r, const double *g,
                            double *out);
+/* The conical jet (bhx_scene.jet_model 1, no reference counterpart). */
+void oracle_cone_velocity(double a, double r, double th, double gamma, double *u);
+void oracle_cone_shape(const bhx_scene *sc, int64_t n, const double *r, const double *cos_th, double *out);
+void oracle_cone_gate(const bhx_scene *sc, int64_t n, const double *c, uint8_t *out);
+double oracle_redshift(const double *y, const double *u);
+double oracle_jet_boost(double alpha, double g);
 void oracle_blackbody_xyz(double T, double *xyz);
 void oracle_blackbody_xyz_exact(double T, double *xyz);
 void oracle_powerlaw_xyz(double alpha, double *xyz);
