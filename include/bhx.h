@@ -331,7 +331,10 @@ int bhx_adjoint_info(bhx_adjoint *adj, int64_t *ck_bytes, int64_t *lit_rays, int
 /* ---- scene preparation (host only, no context, no GPU) ------------------ */
 
 /* Normalise the jets of a scene from its spin and accretion rate: the CLI's
- * setup_jet (src/main.cpp:399-423) with Jet::configure (src/jet.h:261-311).
+ * setup_jet (src/main.cpp:399-423) with Jet::configure (src/jet.h:261-311) -
+ * the radiated fraction of the Blandford-Znajek power spread over the jets'
+ * emissivity-weighted volume, here integrated by Gauss-Legendre quadrature
+ * (the reference's midpoint sums give a j_nu0 0.7-1.7e-8 relative apart).
  * phi_flux is the dimensionless horizon flux (Jet::phi_flux, default 50) and
  * eps_rad the radiated fraction of the jet power (Jet::eps_rad, default 0.02),
  * src/jet.h:140-141, the CLI's --jet-flux and --jet-efficiency.
@@ -348,9 +351,15 @@ int bhx_setup_jet(bhx_scene *scene, double phi_flux, double eps_rad);
 int bhx_setup_conical_jet(bhx_scene *scene, double phi_flux, double eps_rad);
 
 /* Stable-branch star of gravitational mass M_kg for the polytrope
- * P = K_si rho^Gamma (SI units): ns::star_of_mass (src/tov.h:205-218) over
- * ns::solve_tov (src/tov.h:65-177).  Returns BHX_E_INVALID when the equation
- * of state cannot hold that mass. */
+ * P = K_si rho^Gamma (SI units, rho the mass-energy density): what
+ * ns::star_of_mass (src/tov.h:205-218) over ns::solve_tov (src/tov.h:65-177)
+ * gives, central densities searched in the same range [2e17, 5e18] kg m^-3
+ * and the surface at the same P = 1e-10 P_c.  The structure is integrated in
+ * the enthalpy form (converged to ~2e-10 in R; the reference's radius is
+ * 4-9e-8 smaller, its RK4 truncation error) and the stable branch is ended at
+ * the true mass maximum rather than the best of 40 sampled densities.
+ * Returns BHX_E_INVALID when no star of the range has that mass (heavier
+ * than the maximum, or lighter than the star at 2e17 kg m^-3). */
 int bhx_polytrope_star(double K_si, double Gamma, double M_kg, double *R_m, double *M_out);
 
 /* Make a scene a neutron-star scene for a solved star (R_m metres, M_kg kg):

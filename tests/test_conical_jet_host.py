@@ -10,7 +10,8 @@ Blandford-Znajek power over both cones: 4 pi j_nu0 nu_0^alpha band V = L_rad
   in test_host_physics): both take L_rad from the same Blandford-Znajek power,
   and neither volume depends on spin or rate;
 * the volume integral done independently by scipy's adaptive quadrature of
-  the same profile (the library sums on a midpoint grid);
+  the same profile (the library takes the angular part in closed form and
+  the radial part by Gauss-Legendre quadrature);
 * the band factor against scipy's quadrature of nu^-alpha.
 """
 import math
@@ -88,8 +89,7 @@ def test_radiated_power_identity():
     x = om * om
     P_bz = 0.053 / (4 * math.pi) * 50.0 ** 2 * x * (1 + 1.38 * x - 9.2 * x * x) * mdot_c2
     L = 4 * math.pi * sc.jet_j_nu0 * oracle.jet_band_factor(sc.jet_alpha) * V * r_g ** 2
-    # midpoint sums of 400 x 8000 cells against adaptive quadrature
-    assert L == pytest.approx(0.02 * P_bz, rel=2e-6)
+    assert L == pytest.approx(0.02 * P_bz, rel=1e-11)
     assert sc.jet_r_bound == pytest.approx(1.05 * r1, rel=1e-15)
     assert sc.jet_model == 1 and sc.jet_enabled == 1
 
