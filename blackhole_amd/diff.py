@@ -49,13 +49,21 @@ def with_params(scene, p):
 
 
 class DiscRender:
-    """One bhx_adjoint (checkpoint slab, scratch) for frames of one size."""
+    """One bhx_adjoint (checkpoint slab, scratch) for frames of one size.
+
+    The library keeps the checkpoints of the LAST forward only; backward_raw
+    differentiates that one.  Every forward_raw bumps `generation`, and the
+    autograd path (calling the object) records the generation and the
+    parameters of its forward, so that a backward whose checkpoints were
+    overwritten by a later forward first records its own forward again (the
+    forward is deterministic: the same bits, so the same gradient)."""
 
     def __init__(self, tracer, scene, K=64):
         self.tr = tracer
         self.lib = tracer.lib
         self.scene = scene
         self.K = int(K)
+        self.generation = 0
         self.adj = C.c_void_p()
         native.check(tracer.ctx, self.lib.bhx_adjoint_create(tracer.ctx, C.byref(scene), self.K,
                                                              C.byref(self.adj)), "bhx_adjoint_create")
@@ -81,10 +89,12 @@ class DiscRender:
             out = t.empty(s.n_rays, dtype=t.float64, device=f"cuda:{self.tr.device}")
         native.check(self.tr.ctx, self.lib.bhx_adjoint_forward(self.adj, C.c_void_p(out.data_ptr()),
                                                                self.tr._stream(None)), "bhx_adjoint_forward")
+        self.generation += 1
         return out
 
     def backward_raw(self, w):
-        """dL/d(spin, cam_theta, eddington, cam_r) for w = dL/dI per sample."""
+        """dL/d(spin, cam_theta, eddington, cam_r) for w = dL/dI per sample,
+        at the parameters of the last forward_raw."""
         w = w.contiguous()
         g = (C.c_double * 4)()
         native.check(self.tr.ctx, self.lib.bhx_adjoint_backward(self.adj, C.c_void_p(w.data_ptr()), g,
@@ -124,15 +134,21 @@ import torch  # noqa: E402  (torch is the autograd host; the arithmetic is in li
 class _DiscImage(torch.autograd.Function):
     @staticmethod
     def forward(ctx, p, dr):
-        I = dr.forward_raw(p.detach().cpu().numpy())
+        params = p.detach().cpu().numpy().copy()
+        I = dr.forward_raw(params)
         ctx.dr = dr
         ctx.device = p.device
+        ctx.params = params
+        ctx.generation = dr.generation
         return _image_from_samples(dr.tr, I, dr.scene)
 
     @staticmethod
     def backward(ctx, gI):
         dr = ctx.dr
         sc = dr.scene
+        if dr.generation != ctx.generation:
+            # a later forward replaced this one's checkpoints: record it again
+            dr.forward_raw(ctx.params)
         S2 = max(1, sc.sqrt_spp) ** 2
         w = (gI.to(torch.float64).unsqueeze(-1).expand(sc.height, sc.width, S2) / S2).contiguous().view(-1)
         g = dr.backward_raw(w)
