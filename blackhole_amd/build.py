@@ -13,7 +13,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libbhx.so")
 
 HIP_SOURCES = ["tracer.hip", "shade.hip", "api.hip", "order.hip", "batch.hip", "adjoint.hip", "display.hip",
-               "png.hip"]
+               "png.hip", "noalloc.hip"]
 # Per-file flags.  The tracer's loop uses ~70 distinct FP64 constants (DP5
 # tableau, sincos/log/exp coefficients); FP64 VALU ops take no 64-bit
 # literals, so machine LICM would pin every one of them in two registers for
@@ -24,6 +24,7 @@ HIP_SOURCES = ["tracer.hip", "shade.hip", "api.hip", "order.hip", "batch.hip", "
 # the backend, where the choice depends on the surrounding kernel.
 # display.hip: the glare's sums must be the reference's IEEE sequence (no FMA).
 EXTRA = {"tracer.hip": ["-mllvm", "-disable-machine-licm"],
+         "noalloc.hip": ["-ffp-contract=off"],
          "adjoint.hip": ["-mllvm", "-disable-machine-licm", "-ffp-contract=on"],
          "display.hip": ["-ffp-contract=off"]}
 CXX_SOURCES = ["host_setup.cpp", "host_physics.cpp"]

@@ -120,7 +120,11 @@ extern "C" {
 
 int bhx_abi_version(void) { return BHX_ABI_VERSION; }
 
-const char* bhx_last_error(const bhx_context* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+const char* bhx_last_error(const bhx_context* ctx) {
+    if (ctx) return ctx->err.c_str();
+    const char* m = bhx_noalloc_last_error();   // the context-free entry point's
+    return m[0] ? m : "no context";
+}
 
 int bhx_create(int device, bhx_context** out) {
     if (!out) return BHX_E_INVALID;

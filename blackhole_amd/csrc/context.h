@@ -106,6 +106,9 @@ struct bhx_context {
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
+// message of the calling thread's last failed bhx_trace_noalloc_dev (noalloc.hip)
+const char* bhx_noalloc_last_error();
+
 // cost-ordered queue of the current frame into ctx->cost_order / cost_start (order.hip)
 int cost_order(bhx_context* ctx, const bhx::RaygenArgs& G, const bhx::TraceArgs& P, hipStream_t st);
 

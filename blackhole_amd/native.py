@@ -22,7 +22,7 @@ SYMBOLS = (
     "bhx_batch_set_launch", "bhx_batch_get_launch", "bhx_batch_autotune", "bhx_adjoint_create",
     "bhx_adjoint_destroy", "bhx_adjoint_set_scene", "bhx_adjoint_forward", "bhx_adjoint_backward",
     "bhx_adjoint_rays", "bhx_adjoint_copy_rays", "bhx_adjoint_info", "bhx_png_encode", "bhx_png_encode_dev",
-    "bhx_apng_encode", "bhx_apng_encode_dev", "bhx_bytes_free",
+    "bhx_apng_encode", "bhx_apng_encode_dev", "bhx_bytes_free", "bhx_trace_noalloc_dev",
 )
 
 
@@ -85,6 +85,7 @@ def lib():
                                   C.POINTER(Bytes)]
     L.bhx_apng_encode_dev.argtypes = [P, C.POINTER(P), C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P,
                                       C.POINTER(Bytes)]
+    L.bhx_trace_noalloc_dev.argtypes = [C.c_int32, C.POINTER(Scene), C.POINTER(Rays), C.POINTER(Stats), P]
     L.bhx_bytes_free.argtypes = [C.POINTER(Bytes)]
     L.bhx_bytes_free.restype = None
     _lib = L

@@ -184,6 +184,26 @@ int bhx_trace_dev(bhx_context *ctx, const bhx_scene *scene, const bhx_rays *rays
                   bhx_stats *stats, void *stream);
 
 /*
+ * The tracer stage without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same per-ray outputs as bhx_trace_dev, bit for bit, and the same stats
+ * counters (trace_ms and order_ms are left 0), but no context: every scratch
+ * buffer (initial ray states, queue map, tile order and sort workspace,
+ * counters, event parameters) lives in one static device array of about
+ * 140 MiB that each GPU the library runs on holds until the process ends.
+ * The queue is the centre-first tile order, built on the device.
+ *   Limit: width * height * sqrt_spp^2 <= 1048576 samples and at most 32768
+ *   tiles of 8 x 8 pixels; larger frames return BHX_E_TOO_LARGE.
+ * Calls on one GPU share the array: calls that overlap on one GPU race, so
+ * order them on one stream (or synchronise).  `device` is made current.
+ * Failures return a BHX_E_* code; bhx_last_error(NULL) gives the message of
+ * the calling thread's last failure.
+ */
+int bhx_trace_noalloc_dev(int32_t device, const bhx_scene *scene, const bhx_rays *rays, bhx_stats *stats,
+                          void *stream);
+
+/*
  * Shading stage: shade() (src/render.cpp:382-393) of every ray of the frame
  * from the tracer's outputs (cls, y, jet over ALL rays of the scene, ray_begin
  * 0), then the per-pixel average of src/render.cpp:460-462.

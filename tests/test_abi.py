@@ -37,6 +37,23 @@ def test_no_gpu_call_needed_for_errors():
     assert lib.bhx_destroy(None) == 0
 
 
+def test_noalloc_trace_refuses_frames_beyond_its_limit():
+    """bhx_trace_noalloc_dev holds at most 2^20 samples and 2^15 tiles of
+    8 x 8 pixels in its static scratch array; larger frames are refused with
+    BHX_E_TOO_LARGE and a message before any HIP call."""
+    from blackhole_amd import scene as S
+    lib = native.lib()
+    rays = abi.Rays()
+    for w, h, spp in ((1025, 1024, 1), (512, 512, 3), ((1 << 18) + 8, 2, 1)):
+        sc = S.plain_frame(w, h)
+        sc.sqrt_spp = spp
+        assert lib.bhx_trace_noalloc_dev(0, C.byref(sc), C.byref(rays), None, None) == -4, (w, h, spp)
+        assert b"exceeds" in lib.bhx_last_error(None)
+    assert lib.bhx_trace_noalloc_dev(0, None, C.byref(rays), None, None) == -1
+    assert lib.bhx_trace_noalloc_dev(-1, C.byref(S.plain_frame(8, 8)), C.byref(rays), None, None) == -1
+    assert lib.bhx_last_error(None) == b"bhx_trace_noalloc_dev: device index out of range"
+
+
 @pytest.mark.parametrize("struct,cname", [(abi.Scene, "bhx_scene"), (abi.Stats, "bhx_stats"),
                                           (abi.Rays, "bhx_rays"), (abi.Shaded, "bhx_shaded"),
                                           (abi.Launch, "bhx_launch"), (abi.Bytes, "bhx_bytes")])
