@@ -361,6 +361,19 @@ int bhx_polytrope_star(double K_si, double Gamma, double M_kg, double* R_m, doub
     return rc;
 }
 
+int bhx_uniform_sky(int32_t count, uint64_t seed, double* x, double* y, double* z, double* temperature,
+                    double* irradiance) {
+    if (count < 0 || (count > 0 && (!x || !y || !z || !temperature || !irradiance))) return BHX_E_INVALID;
+    bhx::host::Catalogue cat;
+    bhx::host::uniform_sky(count, seed, cat);
+    std::copy(cat.x.begin(), cat.x.end(), x);
+    std::copy(cat.y.begin(), cat.y.end(), y);
+    std::copy(cat.z.begin(), cat.z.end(), z);
+    std::copy(cat.T.begin(), cat.T.end(), temperature);
+    std::copy(cat.irr.begin(), cat.irr.end(), irradiance);
+    return BHX_OK;
+}
+
 int bhx_setup_neutron_star(bhx_scene* s, double R_m, double M_kg) {
     if (!s || !(R_m > 0.0) || !(M_kg > 0.0)) return BHX_E_INVALID;
     const double r_g = kG * M_kg / (kC * kC);

@@ -175,6 +175,31 @@ void generate_catalogue(int count, uint64_t seed, Catalogue& cat) {
     }
 }
 
+void uniform_sky(int count, uint64_t seed, Catalogue& cat) {
+    SplitMix rng{seed};
+    const double k = 0.45, m_lo = -1.5, m_hi = 8.0;
+    const double span = std::pow(10.0, k * (m_hi - m_lo)) - 1.0;
+    const double F0 = 3.64e-9 * 0.089;   // V = 0: 3.64e-9 W m^-2 um^-1 over 0.089 um
+    cat.x.resize(count); cat.y.resize(count); cat.z.resize(count);
+    cat.T.resize(count); cat.irr.resize(count);
+    for (int i = 0; i < count; ++i) {
+        // uniform on the sphere: z uniform on [-1, 1] (Archimedes), phi on [0, 2 pi)
+        const double z = 2.0 * rng.u01() - 1.0;
+        const double ph = 2.0 * kPi * rng.u01();
+        const double rho = std::sqrt(std::max(0.0, (1.0 - z) * (1.0 + z)));
+        cat.x[i] = rho * std::cos(ph);
+        cat.y[i] = rho * std::sin(ph);
+        cat.z[i] = z;
+        // magnitude by inverse CDF of N(<m) ~ 10^(k m) on [m_lo, m_hi]
+        const double m = m_lo + std::log10(1.0 + span * rng.u01()) / k;
+        // B-V ~ N(0.6, 0.4) by Box-Muller (one of the pair)
+        const double radius = std::sqrt(-2.0 * std::log(1.0 - rng.u01()));
+        const double bv = 0.6 + 0.4 * radius * std::cos(2.0 * kPi * rng.u01());
+        cat.T[i] = bv_to_T(bv);
+        cat.irr[i] = F0 * std::pow(10.0, -0.4 * m);
+    }
+}
+
 void bin_catalogue(const Catalogue& cat, std::vector<uint32_t>& cell_start, std::vector<double>& rows) {
     const int n_lat = 360, n_lon = 720, n_cells = n_lat * n_lon;
     const size_t n = cat.x.size();

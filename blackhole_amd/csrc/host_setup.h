@@ -28,6 +28,12 @@ struct Catalogue {
 };
 // The reference's StarField(count, seed) generator (src/scene.h:62-107).
 void generate_catalogue(int count, uint64_t seed, Catalogue& cat);
+// A synthetic sky of `count` stars: directions uniform on the sphere,
+// apparent magnitude m by inverse CDF of N(<m) ~ 10^(0.45 m) on [-1.5, 8],
+// B-V ~ N(0.6, 0.4) (Box-Muller) to colour temperature by Ballesteros (2012),
+// irradiance F0 10^(-0.4 m) with F0 = 3.64e-9 W m^-2 um^-1 x 0.089 um; a
+// splitmix64 stream seeded with `seed`, five draws per star in that order.
+void uniform_sky(int count, uint64_t seed, Catalogue& cat);
 // Cell index on 360 x 720 lat-lon bins (src/scene.h:173-181): stars sorted by
 // cell, ascending index within a cell; rows of 6 doubles x y z irr X/Y Z/Y.
 void bin_catalogue(const Catalogue& cat, std::vector<uint32_t>& cell_start,

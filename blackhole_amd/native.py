@@ -22,7 +22,7 @@ SYMBOLS = (
     "bhx_batch_set_launch", "bhx_batch_get_launch", "bhx_batch_autotune", "bhx_adjoint_create",
     "bhx_adjoint_destroy", "bhx_adjoint_set_scene", "bhx_adjoint_forward", "bhx_adjoint_backward",
     "bhx_adjoint_rays", "bhx_adjoint_copy_rays", "bhx_adjoint_info", "bhx_png_encode", "bhx_png_encode_dev",
-    "bhx_apng_encode", "bhx_apng_encode_dev", "bhx_bytes_free", "bhx_trace_noalloc_dev",
+    "bhx_apng_encode", "bhx_apng_encode_dev", "bhx_bytes_free", "bhx_trace_noalloc_dev", "bhx_uniform_sky",
 )
 
 
@@ -59,6 +59,7 @@ def lib():
     L.bhx_polytrope_star.argtypes = [C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]
     L.bhx_setup_neutron_star.argtypes = [C.POINTER(Scene), C.c_double, C.c_double]
+    L.bhx_uniform_sky.argtypes = [C.c_int32, C.c_uint64, P, P, P, P, P]
     L.bhx_setup_conical_jet.argtypes = [C.POINTER(Scene), C.c_double, C.c_double]
     L.bhx_batch_create.argtypes = [P, C.POINTER(Scene), C.c_int32, C.POINTER(P)]
     L.bhx_batch_destroy.argtypes = [P]

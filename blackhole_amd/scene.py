@@ -97,6 +97,30 @@ def starry_frame(width=1024, height=1024, stars=True):
                        fov=2.0 * math.atan(0.055), disc_r_out=50.0, eddington=0.3, stars=stars)
 
 
+def uniform_sky(count=250000, seed=20240921):
+    """The synthetic sky of the 1024^2 frame: count stars uniform on the
+    sphere, magnitudes from N(<m) ~ 10^(0.45 m) up to m = 8, B-V ~ N(0.6, 0.4)
+    (bhx_uniform_sky).  Returns host arrays (x, y, z, T, irradiance) for
+    Tracer.set_catalogue."""
+    import numpy as np
+    from . import native
+    arrs = [np.empty(count) for _ in range(5)]
+    native.check(None, native.lib().bhx_uniform_sky(int(count), int(seed),
+                                                    *[a.ctypes.data_as(C.c_void_p) for a in arrs]),
+                 "bhx_uniform_sky")
+    return arrs
+
+
+def sky_frame(width=1024, height=1024):
+    """The 1024^2 frame as configured: a = 0.998, 60 deg, r = 1000 M, NT disc
+    ISCO-50 M at 0.3 Eddington, +-55 M image plane, 10 Msun, rtol 1e-10 /
+    atol 1e-12 (the tolerances of the 256^2 frame), stars on; its sky is the
+    catalogue of uniform_sky(250000, seed), loaded with Tracer.set_catalogue."""
+    return plain_scene(spin=0.998, incl_deg=60.0, r_cam=1000.0, width=width, height=height,
+                       fov=2.0 * math.atan(0.055), disc_r_out=50.0, eddington=0.3, stars=True,
+                       rtol=1e-10, atol=1e-12)
+
+
 def setup_jet(s, phi_flux=50.0, eps_rad=0.02):
     """Normalise the scene's jets in place (src/main.cpp:399-423, src/jet.h:261-311);
     clears jet_enabled where the reference's CLI switches the jets off.  phi_flux

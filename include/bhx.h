@@ -382,6 +382,18 @@ int bhx_setup_conical_jet(bhx_scene *scene, double phi_flux, double eps_rad);
  * than the maximum, or lighter than the star at 2e17 kg m^-3). */
 int bhx_polytrope_star(double K_si, double Gamma, double M_kg, double *R_m, double *M_out);
 
+/* A synthetic star catalogue (no counterpart in the reference, whose
+ * StarField puts 45% of its stars near a galactic plane and goes to m = 12):
+ * `count` stars with directions uniform on the sphere, apparent magnitude m
+ * by inverse CDF of N(<m) ~ 10^(0.45 m) on [-1.5, 8] (the reference's count
+ * law, src/scene.h:86-89), B-V ~ N(0.6, 0.4) mapped to colour temperature by
+ * spec::bv_to_temperature (src/spectrum.cpp:158-162), irradiance
+ * F0 10^(-0.4 m) with the reference's F0 (src/scene.h:100-101); one splitmix64
+ * stream seeded with `seed`.  Fills count entries of each host array, ready
+ * for bhx_set_catalogue. */
+int bhx_uniform_sky(int32_t count, uint64_t seed, double *x, double *y, double *z, double *temperature,
+                    double *irradiance);
+
 /* Make a scene a neutron-star scene for a solved star (R_m metres, M_kg kg):
  * the mass and spin rules of the CLI's setup_neutron_star (src/main.cpp:
  * 503-512) and NeutronStar::set_from / set_spin (src/neutronstar.h:69-79);

@@ -122,6 +122,16 @@ def generate_catalogue(count, seed):
     return arrs
 
 
+def uniform_sky(count, seed):
+    """The synthetic uniform sky (the oracle's restatement of bhx_uniform_sky)."""
+    arrs = [np.empty(count) for _ in range(5)]
+    L = lib()
+    L.oracle_uniform_sky.argtypes = [C.c_int32, C.c_uint64, P, P, P, P, P]
+    if L.oracle_uniform_sky(count, seed, *[_ptr(a) for a in arrs]) != 0:
+        raise ValueError("bad star count")
+    return arrs
+
+
 def render(scene, cat=None, threads=0):
     """Whole-frame render (CPU).  Returns (xyz[H, W, 3], Stats)."""
     xyz = np.zeros((scene.height, scene.width, 3))

@@ -538,6 +538,29 @@ void generate_sky(int count, uint64_t seed, Sky& sk) {  // scene.h:62-107
     sk.bin_all();
 }
 
+// The synthetic uniform sky of the 1024^2 frame (the library's
+// bhx_uniform_sky; the reference has no such generator): uniform directions,
+// m from N(<m) ~ 10^(0.45 m) on [-1.5, 8], B-V ~ N(0.6, 0.4), the reference's
+// F0 and B-V -> T mapping; five splitmix64 draws per star.
+void uniform_sky(int count, uint64_t seed, Sky& sk) {
+    SplitMix rng(seed);
+    sk.x.resize(count); sk.y.resize(count); sk.z.resize(count);
+    sk.T.resize(count); sk.irr.resize(count);
+    const double span = std::pow(10.0, 0.45 * (8.0 - (-1.5))) - 1.0;
+    for (int i = 0; i < count; ++i) {
+        const double z = 2.0 * rng.u01() - 1.0;
+        const double ph = 2.0 * kPi * rng.u01();
+        const double rr = std::sqrt(std::max(0.0, (1.0 - z) * (1.0 + z)));
+        sk.x[i] = rr * std::cos(ph);
+        sk.y[i] = rr * std::sin(ph);
+        sk.z[i] = z;
+        const double mag = -1.5 + std::log10(1.0 + span * rng.u01()) / 0.45;
+        const double g = std::sqrt(-2.0 * std::log(1.0 - rng.u01()));
+        sk.T[i] = bv_temperature(0.6 + 0.4 * g * std::cos(2.0 * kPi * rng.u01()));
+        sk.irr[i] = 3.64e-9 * 0.089 * std::pow(10.0, -0.4 * mag);
+    }
+}
+
 void milky_way(const double d[3], double o[3]) {  // scene.h:202-223
     const double ca = 0.8660254037844387, sa = 0.5;
     const double g[3] = {d[0], ca * d[1] + sa * d[2], -sa * d[1] + ca * d[2]};
@@ -1546,6 +1569,18 @@ void oracle_disc_intensity(const bhx_scene* s, int64_t n, const uint8_t* cls, co
         out[i] = kSigmaSB * gT * gT * gT * gT / kPi;
     }
 }
+int oracle_uniform_sky(int32_t count, uint64_t seed, double* x, double* y, double* z, double* T, double* irr) {
+    if (count < 0) return -1;
+    Sky sk;
+    uniform_sky(count, seed, sk);
+    std::copy(sk.x.begin(), sk.x.end(), x);
+    std::copy(sk.y.begin(), sk.y.end(), y);
+    std::copy(sk.z.begin(), sk.z.end(), z);
+    std::copy(sk.T.begin(), sk.T.end(), T);
+    std::copy(sk.irr.begin(), sk.irr.end(), irr);
+    return 0;
+}
+
 // ---- conical jet pieces (this package's own jet model), for the checks ----
 // Four-velocity of the jet's radial outflow at (a, r, theta) for a Lorentz
 // factor gamma relative to the zero-angular-momentum observer.

@@ -94,6 +94,9 @@ void oracle_star_radiance(const bhx_scene *sc, const oracle_catalogue *cat, cons
 /* The reference's StarField(count, seed) catalogue, in generation order. */
 int oracle_generate_catalogue(int32_t count, uint64_t seed, double *x, double *y, double *z,
                               double *temperature, double *irradiance);
+/* The synthetic uniform sky of the 1024^2 frame (bhx_uniform_sky's table). */
+int oracle_uniform_sky(int32_t count, uint64_t seed, double *x, double *y, double *z, double *temperature,
+                       double *irradiance);
 double oracle_auto_exposure(const double *xyz, int64_t n_pixels, double key);
 /* img::add_glare (image.cpp:47-106) on a row-major w x h XYZ image, in place. */
 void oracle_add_glare(double *xyz, int32_t w, int32_t h, double strength, double radius_px);
