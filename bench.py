@@ -1,9 +1,12 @@
-"""Benchmark: plain FP64 Kerr frame at 1024x1024, a=0.998, 250k stars.
+"""Benchmark: plain FP64 Kerr frame at 1024x1024, a=0.998, 250k-star sky.
 
 One step = one pass of the hot path over one frame: the persistent tracer over
 all 1,048,576 rays plus the shading pass (disc emission, star-cell lookup,
 colour), inputs (scene, star catalogue, colour table) already resident on the
-GPU.  Prints one JSON line (the fields are defined where they are computed below).
+GPU.  The frame: a = 0.998, inclination 60 deg, camera at 1000 M, +-55 M image
+plane, NT disc ISCO-50 M at 0.3 Eddington, 250,000 synthetic stars uniform on
+the sphere (m <= 8, B-V ~ N(0.6, 0.4)), rtol 1e-10 / atol 1e-12.  Prints one
+JSON line (the fields are defined where they are computed below).
 """
 import argparse
 import json
@@ -28,21 +31,41 @@ ALG_FLOPS_PER_REJECTED_ATTEMPT = 1417
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector peak (AMD published; FP64 matrix peak is the same)
 # HBM bytes per k_trace launch at this workload, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs scaled by a copy kernel of known size (the file says how).
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_trace_starry_frame_1024_traffic.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_trace_sky_frame_1024_traffic.txt")
+# Executed FP64 operations per k_trace launch at this workload, from the
+# rocprofv3 counters SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 (FMA = 2) scaled
+# to lanes by the measured lane utilisation (the file says how).
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "k_trace_sky_frame_1024_counters.txt")
+
+
+def profile_values(path):
+    """key = value lines of a profile summary, as floats ({} if absent)."""
+    vals = {}
+    try:
+        with open(path) as f:
+            for ln in f:
+                if "=" in ln and not ln.lstrip().startswith("#"):
+                    k, v = ln.split("=", 1)
+                    try:
+                        vals[k.strip()] = float(v.split()[0])
+                    except (ValueError, IndexError):
+                        pass
+    except OSError:
+        pass
+    return vals
 
 
 def traffic_per_launch():
     """HBM bytes per k_trace launch recorded in TRAFFIC_FILE, or None."""
-    try:
-        vals = {}
-        with open(TRAFFIC_FILE) as f:
-            for ln in f:
-                if "=" in ln and not ln.lstrip().startswith("#"):
-                    k, v = ln.split("=", 1)
-                    vals[k.strip()] = float(v.split()[0])
-        return vals["fetch_bytes_per_launch"] + vals["write_bytes_per_launch"]
-    except (OSError, KeyError, ValueError, IndexError):
-        return None
+    v = profile_values(TRAFFIC_FILE)
+    if "fetch_bytes_per_launch" in v and "write_bytes_per_launch" in v:
+        return v["fetch_bytes_per_launch"] + v["write_bytes_per_launch"]
+    return None
+
+
+def executed_flops_per_launch():
+    """Counter-measured FP64 operations per k_trace launch (COUNTERS_FILE), or None."""
+    return profile_values(COUNTERS_FILE).get("executed_fp64_flops_per_launch")
 
 
 def copy_calibration(torch, dev, nbytes=1 << 28, reps=3):
@@ -67,14 +90,15 @@ def cpu_model():
     return platform.processor() or platform.machine()
 
 
-def cpu_baseline(sc, threads):
+def cpu_baseline(sc, threads, cat=None):
     """The oracle (CPU restatement of the reference) on every ray of the same
-    frame with `threads` host threads (the reference's row-pool model)."""
+    frame with `threads` host threads (the reference's row-pool model); `cat`
+    an oracle.Cat star catalogue (None: the reference's generator)."""
     import numpy as np
     import oracle
     ids = np.arange(sc.n_rays)
     t0 = time.perf_counter()
-    out = oracle.trace_list(sc, ids, threads=threads)
+    out = oracle.trace_list(sc, ids, cat=cat, threads=threads)
     dt = time.perf_counter() - t0
     steps = int(out["steps"].sum())
     att = int(out["attempts"].sum())
@@ -118,6 +142,39 @@ def small_frame_leg(tr, torch, threads, rounds=20, cpu=True):
     if cpu:
         res["cpu_baseline"] = cpu_baseline(sc, threads)
     return res
+
+
+def cli_frame_leg(tr, torch, steps=20, warmup=5):
+    """The same 1024^2 frame as the reference's CLI renders it: its default
+    tolerances (rtol 1e-9 / atol 1e-11) and its own star generator
+    (StarField(250000, seed): 45% of the stars near a galactic plane, m up to
+    12).  Trace + shade per step; reported beside the headline."""
+    from blackhole_amd import scene as S
+    sc = S.starry_frame(1024, 1024)
+    tr.set_catalogue()     # the reference's generator
+    rays = tr.alloc_rays(sc, full=False)
+    out = {"xyz": torch.empty((sc.height, sc.width, 3), dtype=torch.float64, device="cuda:0")}
+    for _ in range(warmup):
+        tr.trace_async(sc, rays)
+        tr.shade_async(sc, rays, out)
+    _, st = tr.trace(sc, rays)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        tr.trace_async(sc, rays)
+        tr.shade_async(sc, rays, out)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    kern = [tr.trace(sc, rays)[1].trace_ms for _ in range(5)]
+    rejected = st.attempts - st.steps
+    flops = ALG_FLOPS_PER_ACCEPTED_STEP * st.steps + ALG_FLOPS_PER_REJECTED_ATTEMPT * rejected
+    trace_ms = sum(kern) / len(kern)
+    return {"workload": "plain Kerr frame 1024x1024, a=0.998, i=60 deg, r_cam=1000M, +-55M, NT disc ISCO-50M "
+                        "@0.3 Edd, 250k stars from the reference's StarField generator, rtol 1e-9 / atol 1e-11 "
+                        "(the reference CLI's defaults), 1 spp",
+            "ms_per_frame": round(ms, 4), "mrays_s": round(sc.n_rays / (ms * 1e-3) / 1e6, 3),
+            "trace_ms": round(trace_ms, 4), "accepted_steps_per_ray": st.steps / sc.n_rays,
+            "k_trace_frac_fp64_peak": round(flops / (trace_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4)}
 
 
 def batched_leg(tr, torch, rounds=3):
@@ -290,8 +347,9 @@ def sweep_leg(tr, torch, n_frames=120, width=1024, K=64, rounds=2, grad=True, fd
     frames = S.quasar_sweep(n_frames, width, width)
     n_rays = sum(sc.n_rays for sc in frames)
     res = {"workload": f"{n_frames} frames {width}x{width}: a=0.998, inclination 0.5->90 deg, NT disc "
-                       f"ISCO-1000M @0.5 Edd, conical jets 5 deg Gamma 10 (r0/r)^2 6-500M, 250k stars, "
-                       f"r_cam 2500M, rtol 1e-9 / atol 1e-11; one batched launch",
+                       f"ISCO-1000M @0.5 Edd, conical jets 5 deg Gamma 10 (r0/r)^2 6-500M, 250k stars "
+                       f"uniform on the sphere (the headline frame's sky), r_cam 2500M, rtol 1e-9 / "
+                       f"atol 1e-11; one batched launch",
            "frames": n_frames, "rays": n_rays}
     with Batch(tr, frames) as b:
         b.run()
@@ -399,6 +457,8 @@ def main():
     ap.add_argument("--no-sweep", action="store_true", help="skip the quasar-sweep leg")
     ap.add_argument("--no-png", action="store_true", help="skip the PNG-writer leg")
     ap.add_argument("--no-small-frame", action="store_true", help="skip the 256x256 plain-frame leg")
+    ap.add_argument("--no-cli-frame", action="store_true",
+                    help="skip the frame at the CLI's tolerances with the reference's star generator")
     ap.add_argument("--sweep-frames", type=int, default=120)
     ap.add_argument("--sweep-width", type=int, default=1024)
     ap.add_argument("--no-sweep-grad", action="store_true", help="sweep leg without the per-frame gradients")
@@ -413,11 +473,13 @@ def main():
     import torch
     from blackhole_amd import Tracer, scene as S
 
-    sc = S.starry_frame(args.width, args.width)
+    sc = S.sky_frame(args.width, args.width)
+    sky = S.uniform_sky(sc.star_count, sc.seed)
     dev = torch.device("cuda:0")
     if args.copy_calibration:
         copy_calibration(torch, dev)
     tr = Tracer(0)
+    tr.set_catalogue(*sky)
     rays = tr.alloc_rays(sc, full=False)
     out = {"xyz": torch.empty((sc.height, sc.width, 3), dtype=torch.float64, device=dev)}
     # warm-up: uploads the star catalogue, colour table and tile order
@@ -455,6 +517,18 @@ def main():
         order.append(sk.order_ms)
     trace_ms = sum(kern) / reps
     order_ms = sum(order) / reps
+    # end to end: the frame to 8-bit sRGB on the device (trace, shade,
+    # auto exposure - a radix selection with host round trips - and develop)
+    e2e_steps = max(1, min(args.steps, 10))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(e2e_steps):
+        tr.trace_async(sc, rays)
+        tr.shade_async(sc, rays, out)
+        rgb = tr.develop(out["xyz"], tr.auto_exposure(out["xyz"]))
+    torch.cuda.synchronize()
+    e2e_ms = (time.perf_counter() - t0) / e2e_steps * 1e3
+    del rgb
 
     ms_per_step = wall / args.steps * 1e3
     n_rays = sc.n_rays
@@ -462,7 +536,11 @@ def main():
     rejected = st.attempts - st.steps
     alg_flops = ALG_FLOPS_PER_ACCEPTED_STEP * st.steps + ALG_FLOPS_PER_REJECTED_ATTEMPT * rejected
     achieved = alg_flops / (trace_ms * 1e-3) / 1e12
-    rhs = 6 * st.attempts + n_rays
+    # RHS evaluations: 6 per attempt, the first stage of each ray (k_raygen)
+    # and one at the end of each escaped ray (its escape direction, k_shade)
+    rhs = 6 * st.attempts + n_rays + st.escaped
+    ex_flops = executed_flops_per_launch()
+    executed = ex_flops / (trace_ms * 1e-3) / 1e12 if ex_flops else None
     line = {
         "metric": "Mrays/s + DP5 RHS evals/s at 1024^2 a=0.998 (FP64); batched/plain; grad/fwd time",
         "value": round(mrays, 4),
@@ -477,9 +555,12 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"plain Kerr frame {sc.width}x{sc.height}, a=0.998, i=60 deg, "
-                               f"r_cam=1000M, NT disc ISCO-50M @0.3 Edd, 250k stars "
-                               f"(reference generator), rtol 1e-9 / atol 1e-11, 1 spp"},
+                               f"r_cam=1000M, +-55M image plane, 10 Msun, NT disc ISCO-50M @0.3 Edd, "
+                               f"250k stars uniform on the sphere (m<=8, B-V~N(0.6,0.4), seed "
+                               f"{sc.seed}), rtol 1e-10 / atol 1e-12, 1 spp"},
         "rhs_evals_per_s": rhs / (ms_per_step * 1e-3),
+        "end_to_end_ms": round(e2e_ms, 4),
+        "end_to_end_mrays_s": round(n_rays / (e2e_ms * 1e-3) / 1e6, 3),
         "accepted_steps_per_ray": st.steps / n_rays,
         "rejected_attempts": int(rejected),
         "trace_ms": round(trace_ms, 4),
@@ -489,16 +570,22 @@ def main():
         "shade_ms": round(st.shade_ms, 4),
         "batched_over_plain": None,
         "grad_over_fwd": None,
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+        "roofline": {"bound": "fp64-vector", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                      "traffic": traffic_per_launch(), "kernel": "k_trace",
                      "pipe": "FP64 compute-bound on the vector ALUs: no MFMA (no dense "
                              "contraction on this path); 78.6 TFLOP/s is MI355X's FP64 vector "
                              "peak, equal to its FP64 matrix peak",
-                     "flops_per_launch": alg_flops},
+                     "flops_per_launch": alg_flops,
+                     "executed": None if executed is None else round(executed, 3),
+                     "executed_frac": None if executed is None else round(executed / FP64_PEAK_TFLOPS, 4),
+                     "executed_flops_per_launch": ex_flops},
     }
     if not args.no_png:
         line["png"] = png_leg(tr, torch, sc, out["xyz"], cpu=not args.no_cpu)
+    if not args.no_cli_frame:
+        line["cli_frame"] = cli_frame_leg(tr, torch)
+        tr.set_catalogue(*sky)
     if not args.no_small_frame:
         line["small_frame"] = small_frame_leg(tr, torch, min(16, os.cpu_count() or 1), cpu=not args.no_cpu)
     if not args.no_batch:
@@ -512,8 +599,9 @@ def main():
         line["sweep"] = sweep_leg(tr, torch, n_frames=nf, width=args.sweep_width, grad=not args.no_sweep_grad,
                                   fd_frames=tuple(sorted({0, nf // 3, (2 * nf) // 3, nf - 1})))
     if not args.no_cpu:
+        import oracle
         threads = min(16, os.cpu_count() or 1)
-        line["cpu_baseline"] = cpu_baseline(sc, threads)
+        line["cpu_baseline"] = cpu_baseline(sc, threads, cat=oracle.Cat(*oracle.uniform_sky(sc.star_count, sc.seed)))
     print(json.dumps(line), flush=True)
 
 

@@ -1,8 +1,11 @@
 """HIP tracer vs the CPU oracle on identical scenes (needs an MI355X).
 
-The plain 256x256 frame (a=0.9, rtol 1e-10) is checked ray by ray over the whole
-frame; the starry 1024x1024 frame (a=0.998, 250k stars) over a strided sample of
-rays plus whole-frame properties.  The bar is the one of tests/parity.py.
+Checked ray by ray over the whole frame: the plain 256x256 frame (a=0.9,
+rtol 1e-10), the 1024x1024 frame as configured (a=0.998, 250k stars uniform
+on the sphere, rtol 1e-10) and the same frame as the reference's CLI renders
+it (its star generator, rtol 1e-9), plus whole-frame properties (exposure,
+sRGB, per-class counts, RenderStats::total_flux).  The bar is the one of
+tests/parity.py.
 """
 import math
 
@@ -60,24 +63,60 @@ def test_plain_frame_256_full(tracer):
     assert np.mean(g["steps"][same] == c["steps"][same]) > 0.99
 
 
-def test_starry_frame_1024_sampled(tracer):
-    sc = S.starry_frame()
-    print("[starry 1024] 1024x1024 a=0.998 250k stars", flush=True)
+def _full_frame(tracer, sc, name, cat=None):
+    """Every ray of a frame against the oracle, and the frame's display and
+    RenderStats against the oracle's."""
     rays, sh, st = _run(tracer, sc)
-    ids = np.arange(0, sc.n_rays, 37)
-    g = {k: v[..., ids] for k, v in gpu_rays_numpy(rays, sh).items()}
-    c = oracle.trace_list(sc, ids)
-    rep = compare(sc, ids, g, c)
-    _report("starry1024", rep)
+    ids = np.arange(sc.n_rays)
+    g = gpu_rays_numpy(rays, sh)
+    c = oracle.trace_list(sc, ids, cat=cat)
+    rep = compare(sc, ids, g, c, cat=cat)
+    _report(name, rep)
     assert rep["fail"] == 0, rep
     assert rep["exempt"] <= 0.002 * rep["n"], rep
-    # colour of every sampled ray: within 1e-6 relative (a star's Gaussian PSF
-    # turns a 1e-12 direction difference into ~1e-7 of its radiance)
+    # colour of every ray: within 1e-6 relative (a star's Gaussian PSF turns a
+    # 1e-12 direction difference into ~1e-7 of its radiance)
     ok = g["cls"] == c["cls"]
     rel = np.abs(g["ray_xyz"] - c["ray_xyz"]) / np.maximum(np.abs(c["ray_xyz"]), 1e-300)
     bad = (rel > 1e-6).any(axis=0) & ok & (c["ray_xyz"][1] > 0)
     assert bad.sum() <= rep["exempt"] + 2, np.nonzero(bad)[0][:10]
+    # the frame: one exposure, sRGB within 1 LSB, per-class counts, and
+    # total_flux = sum of the samples' weighted Y (src/render.cpp:460-462)
+    xyz_g = sh["xyz"].cpu().numpy()
+    xyz_c = c["ray_xyz"].T.reshape(sc.height, sc.width, 3)
+    e_c = oracle.auto_exposure(xyz_c)
+    assert abs(tracer.auto_exposure(sh["xyz"]) - e_c) <= 1e-9 * abs(e_c)
+    assert srgb_lsb_diff(xyz_g, xyz_c, e_c) <= 1
+    counts = np.bincount(c["cls"], minlength=6)
+    for cls, got in ((0, st.captured), (1, st.disc_hits), (4, st.escaped), (5, st.exhausted)):
+        assert abs(got - counts[cls]) <= rep["exempt"], (cls, got, counts[cls])
+    flux_c = float(np.sum(c["ray_xyz"][1]))
+    print(f"[{name}] total_flux gpu {st.total_flux:.17g} oracle {flux_c:.17g}", flush=True)
+    assert abs(st.total_flux - flux_c) <= 1e-9 * abs(flux_c)
     assert st.max_norm_error < 1e-8 and st.max_carter_drift < 1e-5
+    return rep
+
+
+def test_sky_frame_1024_full(tracer):
+    """The 1024^2 frame as configured: uniform 250k-star sky, rtol 1e-10."""
+    sc = S.sky_frame()
+    sky = S.uniform_sky(sc.star_count, sc.seed)
+    cat = oracle.Cat(*oracle.uniform_sky(sc.star_count, sc.seed))
+    assert all(np.array_equal(a, b) for a, b in zip(sky, cat.arrays))
+    print("[sky 1024] 1024x1024 a=0.998 uniform 250k-star sky rtol 1e-10", flush=True)
+    tracer.set_catalogue(*sky)
+    try:
+        _full_frame(tracer, sc, "sky1024", cat=cat)
+    finally:
+        tracer.set_catalogue()
+
+
+def test_starry_frame_1024_full(tracer):
+    """The same frame with the reference's StarField generator at the CLI's
+    tolerances (rtol 1e-9 / atol 1e-11)."""
+    sc = S.starry_frame()
+    print("[starry 1024] 1024x1024 a=0.998 reference star generator rtol 1e-9", flush=True)
+    _full_frame(tracer, sc, "starry1024")
 
 
 @pytest.mark.parametrize("kind", ["spp2", "neutron_star", "hotspot"])
