@@ -23,8 +23,12 @@ HIP_SOURCES = ["tracer.hip", "shade.hip", "api.hip", "order.hip", "batch.hip", "
 # per source expression only (-ffp-contract=on), never across expressions by
 # the backend, where the choice depends on the surrounding kernel.
 # display.hip: the glare's sums must be the reference's IEEE sequence (no FMA).
+# shade.hip: the same LICM setting as the tracer's (the shading loop inlines the
+# library's acos / atan2 / exp / log / pow, whose constants would otherwise be
+# hoisted into registers: 256 VGPRs and scratch spills instead of 146 and none).
 EXTRA = {"tracer.hip": ["-mllvm", "-disable-machine-licm"],
          "noalloc.hip": ["-ffp-contract=off"],
+         "shade.hip": ["-mllvm", "-disable-machine-licm"],
          "adjoint.hip": ["-mllvm", "-disable-machine-licm", "-ffp-contract=on"],
          "display.hip": ["-ffp-contract=off"]}
 CXX_SOURCES = ["host_setup.cpp", "host_physics.cpp"]

@@ -161,7 +161,7 @@ int bhx_destroy(bhx_context* ctx) {
     }
     for (DevBuf* b : {&ctx->cell_start, &ctx->star_rows,
                       &ctx->init, &ctx->q2ray, &ctx->evp, &ctx->flux_partial,
-                      &ctx->lum_hist, &ctx->lum_npos, &ctx->r_cls, &ctx->r_y, &ctx->r_jet, &ctx->r_xyz,
+                      &ctx->exp_hist, &ctx->exp_state, &ctx->r_cls, &ctx->r_y, &ctx->r_jet, &ctx->r_xyz,
                       &ctx->pilot_init, &ctx->pilot_q2ray, &ctx->pilot_att, &ctx->pilot_cls, &ctx->order_keys,
                       &ctx->order_vals, &ctx->order_sizes, &ctx->cost_order, &ctx->cost_start, &ctx->order_tmp,
                       &ctx->png_in, &ctx->png_raw, &ctx->png_key, &ctx->png_pos, &ctx->png_prev, &ctx->png_match,
@@ -412,53 +412,25 @@ int bhx_auto_exposure_dev(bhx_context* ctx, const double* xyz, int64_t n, double
     if (!xyz || !exposure || n < 0) return fail(ctx, BHX_E_INVALID, "bad argument");
     HIPCHK(hipSetDevice(ctx->device));
     hipStream_t st = static_cast<hipStream_t>(stream);
-    HIPCHK(ctx->lum_hist.ensure(sizeof(unsigned int) * 65536));
-    HIPCHK(ctx->lum_npos.ensure(sizeof(unsigned long long)));
-    unsigned int* hist = static_cast<unsigned int*>(ctx->lum_hist.p);
-    unsigned long long* npos = static_cast<unsigned long long*>(ctx->lum_npos.p);
-    // count positive finite luminances with the first pass of the first selection
-    std::vector<unsigned int> h(65536);
-    unsigned long long n_pos = 0;
-    auto select = [&](uint64_t k, double* out) -> int {
-        unsigned long long prefix = 0;
-        for (int pass = 0; pass < 4; ++pass) {
-            const int shift = 48 - 16 * pass;
-            HIPCHK(launch_zero_u32(hist, 65536, st));
-            if (pass == 0) HIPCHK(launch_zero_u64(npos, 1, st));
-            HIPCHK(launch_lum_hist(xyz, n, shift, prefix, hist, pass == 0 ? npos : nullptr, st));
-            HIPCHK(hipMemcpyAsync(h.data(), hist, sizeof(unsigned int) * 65536, hipMemcpyDeviceToHost, st));
-            if (pass == 0) HIPCHK(hipMemcpyAsync(&n_pos, npos, sizeof n_pos, hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
-            if (pass == 0 && k == UINT64_MAX) return 1;  // count-only call
-            uint64_t cum = 0;
-            int digit = 0;
-            for (; digit < 65536; ++digit) {
-                if (cum + h[digit] > k) break;
-                cum += h[digit];
-            }
-            if (digit == 65536) return fail(ctx, BHX_E_HIP, "radix select lost the k-th element");
-            k -= cum;
-            prefix = (prefix << 16) | static_cast<unsigned long long>(digit);
-        }
-        std::memcpy(out, &prefix, 8);
-        return 0;
-    };
-    int rc = select(UINT64_MAX, nullptr);
-    if (rc < 0) return rc;
-    if (n_pos == 0) {
+    HIPCHK(ctx->exp_hist.ensure(sizeof(unsigned int) * 2 * 4096));
+    HIPCHK(ctx->exp_state.ensure(sizeof(ExposureState)));
+    unsigned int* hist = static_cast<unsigned int*>(ctx->exp_hist.p);
+    ExposureState* state = static_cast<ExposureState*>(ctx->exp_state.p);
+    // both order statistics (image.cpp:29-36) selected on the device, then
+    // one copy of the selection's state to the host
+    HIPCHK(launch_zero_u32(hist, 2 * 4096, st));
+    HIPCHK(launch_zero_u64(reinterpret_cast<unsigned long long*>(state), sizeof(ExposureState) / 8, st));
+    if (n > 0) HIPCHK(launch_exposure(xyz, n, state, hist, st));
+    ExposureState h{};
+    HIPCHK(hipMemcpyAsync(&h, state, sizeof h, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (h.n_pos == 0) {
         *exposure = 1.0;
         return BHX_OK;
     }
-    // image.cpp:29-44: k = floor(q (n-1)) order statistics, q = 0.999 and 0.9999
-    auto kth = [&](double q) {
-        uint64_t k = static_cast<uint64_t>(q * static_cast<double>(n_pos - 1));
-        return std::min<uint64_t>(k, n_pos - 1);
-    };
-    double hi = 0.0, peak = 0.0;
-    rc = select(kth(0.999), &hi);
-    if (rc) return rc;
-    rc = select(kth(0.9999), &peak);
-    if (rc) return rc;
+    double hi, peak;
+    std::memcpy(&hi, &h.prefix[0], 8);
+    std::memcpy(&peak, &h.prefix[1], 8);
     double e = 1.0;
     if (hi > 0.0) e = key / hi;
     if (peak > 0.0) e = std::min(e, 3.0 * key / peak);

@@ -22,8 +22,8 @@ hipError_t trace_batch_occupancy(int block, int n_scenes, int* blocks_per_cu);
 hipError_t launch_shade(const ShadeArgs& args, int blocks, hipStream_t s);
 hipError_t launch_develop(const double* xyz, int64_t n, double exposure, int tm, double log_k,
                           uint8_t* rgb, hipStream_t s);
-hipError_t launch_lum_hist(const double* xyz, int64_t n, int shift, unsigned long long prefix,
-                           unsigned int* hist, unsigned long long* n_pos, hipStream_t s);
+hipError_t launch_exposure(const double* xyz, int64_t n, ExposureState* state, unsigned int* hist,
+                           hipStream_t s);
 hipError_t launch_bb(const double* tab, const double* T, int64_t n, double* out, hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);
 hipError_t launch_zero_u32(unsigned int* p, int n, hipStream_t s);
@@ -98,7 +98,7 @@ struct bhx_context {
     bool ev_valid = false;
     // counters
     Counters* d_cnt = nullptr;
-    DevBuf flux_partial, lum_hist, lum_npos;
+    DevBuf flux_partial, exp_hist, exp_state;
     // frame buffers of bhx_render
     DevBuf r_cls, r_y, r_jet, r_xyz;
     // veiling glare: row pass, the two blurred images, the taps

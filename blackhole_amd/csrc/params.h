@@ -118,6 +118,15 @@ struct TraceArgs {
     unsigned long long* totals;
 };
 
+// Device state of the exposure's radix selection between its passes: the
+// count of positive finite luminances, and per order statistic its remaining
+// rank and the key bits selected so far.
+struct ExposureState {
+    unsigned long long n_pos;
+    unsigned long long k[2];
+    unsigned long long prefix[2];
+};
+
 // Disc / NS / sky shading constants (src/render.cpp:265-393, src/disc.h,
 // src/scene.h:111-223, src/neutronstar.h:90-125).
 struct ShadeArgs {

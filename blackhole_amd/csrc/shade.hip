@@ -4,13 +4,16 @@
 //              the per-pixel sample average (src/render.cpp:443-462); one
 //              thread per pixel visiting its samples in the reference's order
 //   k_develop  img::develop (src/image.cpp:119-191) to 8-bit sRGB
-//   k_lum_hist radix-select histogram behind img::auto_exposure (image.cpp:22-45)
+//   k_exp_hist / k_exp_select  the device radix selection behind
+//              img::auto_exposure (image.cpp:22-45)
 //   k_bb       spec::blackbody_xyz (src/spectrum.cpp:123-134) of a T array
 #include "kerr_dev.h"
 #include "params.h"
 #include "../../include/bhx.h"
 
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 namespace bhx {
 namespace {
@@ -124,12 +127,12 @@ __device__ void sky_radiance(const ShadeArgs& P, const double d[3], double o[3])
     o[2] += amp * P.mw_cz;
 }
 
-// Two waves per SIMD: unconstrained, the class branches (library acos, atan2,
-// pow, trig) take ~300 registers and one wave per SIMD, which leaves the
-// star-cell gathers' latency bare; capped at 256 the few spills fall in the
-// rare branches (0.29 -> 0.22 ms per 1024^2 frame; 3 waves spill in the
-// common ones and are slower).
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_shade(const ShadeArgs P) {
+// Three waves per SIMD (146 VGPRs, no scratch).  The file is compiled without
+// machine loop-invariant code motion: with it the backend hoists the FP64
+// constants of every inlined acos / atan2 / exp / log / pow out of the
+// per-sample loop into registers for the whole kernel, which took 256 VGPRs
+// and 180 bytes of scratch per lane at two waves per SIMD.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_shade(const ShadeArgs P) {
     __shared__ unsigned long long counts[7];
     __shared__ double flux_w[4];
     __shared__ unsigned long long ttot[4];
@@ -387,27 +390,101 @@ __global__ void k_develop(const double* __restrict__ xyz, int64_t n, double expo
     }
 }
 
-// One radix-select pass over positive finite luminances: histogram of the
-// 16-bit digit at `shift` among keys whose bits above it equal `prefix`.
-// Positive doubles order like their bit patterns.
-__global__ void k_lum_hist(const double* __restrict__ xyz, int64_t n, int shift,
-                           unsigned long long prefix, unsigned int* __restrict__ hist,
-                           unsigned long long* __restrict__ n_pos) {
+// img::auto_exposure's two order statistics (image.cpp:22-45) by radix
+// selection on the bits of the positive finite luminances (positive doubles
+// order like their bit patterns), on the device from start to end: six digit
+// passes (12, 12, 12, 12, 12 and 4 bits from the top), each a histogram pass
+// over the frame and a one-block selection of each statistic's digit.  The
+// state between passes (count, ranks, prefixes) stays in device memory.
+constexpr int kExpBins = 4096;
+__host__ __device__ constexpr int exp_shift(int pass) { return pass < 5 ? 52 - 12 * pass : 0; }
+__host__ __device__ constexpr int exp_width(int pass) { return pass < 5 ? 12 : 4; }
+
+// Histograms of this pass's digit, one per statistic, of the keys whose bits
+// above the digit equal that statistic's prefix; per-block counts in LDS.
+__global__ void __launch_bounds__(256) k_exp_hist(const double* __restrict__ xyz, int64_t n, int pass,
+                                                  const ExposureState* __restrict__ state,
+                                                  unsigned int* __restrict__ hist) {
+    __shared__ unsigned int h[2 * kExpBins];
+    for (int i = threadIdx.x; i < 2 * kExpBins; i += blockDim.x) h[i] = 0u;
+    __syncthreads();
+    const int shift = exp_shift(pass), width = exp_width(pass), top = shift + width;
+    const unsigned long long mask = (1ull << width) - 1ull;
+    const unsigned long long p0 = state->prefix[0], p1 = state->prefix[1];
+    const bool live = pass == 0 || state->n_pos > 0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    unsigned long long cnt = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; live && i < n; i += stride) {
         const double Y = xyz[3 * i + 1];
         if (!(Y > 0.0 && isfinite(Y))) continue;
-        ++cnt;
         const unsigned long long key = (unsigned long long)__double_as_longlong(Y);
-        const int top = shift + 16;
-        if (top < 64 && (key >> top) != prefix) continue;
-        atomicAdd(&hist[(key >> shift) & 0xFFFFu], 1u);
+        const unsigned int digit = (unsigned int)((key >> shift) & mask);
+        if (pass == 0) {
+            atomicAdd(&h[digit], 1u);
+            continue;
+        }
+        const unsigned long long above = key >> top;
+        if (above == p0) atomicAdd(&h[digit], 1u);
+        if (above == p1) atomicAdd(&h[kExpBins + digit], 1u);
     }
-    if (n_pos) {
-        for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
-        if (__lane_id() == 0 && cnt) atomicAdd(n_pos, cnt);
+    __syncthreads();
+    for (int i = threadIdx.x; i < 2 * kExpBins; i += blockDim.x)
+        if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// One block of 256 threads: the digit of each statistic's rank in this pass's
+// histogram (16 bins per thread, a block scan of the per-thread sums), the
+// new prefix and remaining rank; clears the histograms for the next pass.
+// Pass 0 also counts the positive finite luminances n and sets the ranks
+// k = floor(q (n - 1)), q = 0.999 and 0.9999, as the reference does.
+__global__ void __launch_bounds__(256) k_exp_select(int pass, ExposureState* __restrict__ state,
+                                                    unsigned int* __restrict__ hist) {
+    __shared__ unsigned long long part[256];
+    __shared__ unsigned long long total;
+    const int t = threadIdx.x;
+    constexpr int kPer = kExpBins / 256;
+    for (int j = 0; j < 2; ++j) {
+        const unsigned int* hj = hist + (pass == 0 ? 0 : j * kExpBins);
+        unsigned long long mine = 0;
+        for (int b = 0; b < kPer; ++b) mine += hj[t * kPer + b];
+        part[t] = mine;
+        __syncthreads();
+        // inclusive scan of the 256 partial sums (Hillis-Steele)
+        for (int off = 1; off < 256; off <<= 1) {
+            const unsigned long long v = (t >= off) ? part[t - off] : 0ull;
+            __syncthreads();
+            part[t] += v;
+            __syncthreads();
+        }
+        if (pass == 0 && j == 0 && t == 255) {
+            total = part[255];
+            state->n_pos = total;
+            if (total > 0) {
+                for (int q = 0; q < 2; ++q) {
+                    const double frac = q == 0 ? 0.999 : 0.9999;
+                    unsigned long long k = (unsigned long long)(frac * (double)(total - 1));
+                    state->k[q] = k < total - 1 ? k : total - 1;
+                }
+            }
+        }
+        __syncthreads();
+        if (state->n_pos > 0) {
+            const unsigned long long k = state->k[j];
+            const unsigned long long before = part[t] - mine;
+            if (k >= before && k < part[t]) {   // the rank falls in this thread's bins
+                unsigned long long cum = before;
+                int b = 0;
+                for (; b < kPer; ++b) {
+                    const unsigned long long c = hj[t * kPer + b];
+                    if (k < cum + c) break;
+                    cum += c;
+                }
+                state->k[j] = k - cum;
+                state->prefix[j] = (state->prefix[j] << exp_width(pass)) | (unsigned long long)(t * kPer + b);
+            }
+        }
+        __syncthreads();
     }
+    for (int i = t; i < 2 * kExpBins; i += 256) hist[i] = 0u;
 }
 
 __global__ void k_bb(const double* __restrict__ tab, const double* __restrict__ T, int64_t n,
@@ -440,10 +517,19 @@ hipError_t launch_develop(const double* xyz, int64_t n, double exposure, int tm,
     hipLaunchKernelGGL(k_develop, dim3(blocks), dim3(256), 0, s, xyz, n, exposure, tm, log_k, rgb);
     return hipGetLastError();
 }
-hipError_t launch_lum_hist(const double* xyz, int64_t n, int shift, unsigned long long prefix,
-                           unsigned int* hist, unsigned long long* n_pos, hipStream_t s) {
-    hipLaunchKernelGGL(k_lum_hist, dim3(1024), dim3(256), 0, s, xyz, n, shift, prefix, hist, n_pos);
-    return hipGetLastError();
+hipError_t launch_exposure(const double* xyz, int64_t n, ExposureState* state, unsigned int* hist,
+                           hipStream_t s) {
+    const int blocks = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 255) / 256));
+    for (int pass = 0; pass < 6; ++pass) {
+        hipLaunchKernelGGL(k_exp_hist, dim3(blocks), dim3(256), 0, s, xyz, n, pass,
+                           static_cast<const ExposureState*>(state), hist);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_exp_select, dim3(1), dim3(256), 0, s, pass, state, hist);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 hipError_t launch_bb(const double* tab, const double* T, int64_t n, double* out, hipStream_t s) {
     const int blocks = (int)((n + 255) / 256);
