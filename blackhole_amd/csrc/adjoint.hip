@@ -10,8 +10,10 @@
 //                   jet), so a wave holds neighbouring rays of similar cost.  Writes the
 //                   class, hit state, accepted-step count and intensity
 //                   (disc plus jet) of each sample, and a checkpoint of the
-//                   integrator state every K accepted steps (HBM slab of
-//                   cap = max_steps / K + 1 records of 8 doubles per ray).
+//                   integrator state every K accepted steps into a pool of
+//                   8-record chunks that each ray takes as it goes (an atomic
+//                   counter), each chunk linked back to the ray's previous
+//                   one: the pool holds what the rays' real step counts need.
 //   k_adj_backward  one lit ray (I > 0) per lane, sorted by step count so the
 //                   lanes of a wave replay and sweep segments of equal length
 //                   together (a wave's few jet-lit lanes share their jet
@@ -34,12 +36,68 @@ using namespace bhx::adj;
 
 namespace {
 
+// Checkpoint pool: chunks of kCkChunk records of kCk doubles; link[c] is the
+// chunk its ray filled before c (-1 for the ray's first).  Per ray, the last
+// chunk and the number of records.
+constexpr int kCkChunk = 8;
+constexpr int64_t kMaxPoolChunks = (int64_t(128) << 30) / (kCk * kCkChunk * sizeof(double));   // 128 GiB
+
+// The forward's store of one ray: record j goes to slot j % kCkChunk of the
+// ray's current chunk; a new chunk is taken from the counter every kCkChunk
+// records.  Past the pool's end the counter still counts (it tells the host
+// the size needed) and the records are dropped: the ray is marked lost
+// (current chunk kLost).  The lane's current chunk and record count live in
+// an LDS slot (x, y; z holds the lane's ray index): they are touched once
+// every K steps, and registers are what the forward is short of.
+constexpr int32_t kLost = -2;
+struct PoolWriter {
+    double* pool;
+    int32_t* link;
+    uint32_t* counter;
+    uint32_t cap;              // chunks in the pool
+    int4* slot;
+    __device__ void start(int32_t ray) { *slot = make_int4(-1, 0, ray, 0); }
+    __device__ double* put(int j) {
+        int32_t cur = slot->x;
+        if (j % kCkChunk == 0) {
+            const uint32_t c = atomicAdd(counter, 1u);
+            if (c < cap && cur != kLost) {
+                link[c] = cur;
+                cur = static_cast<int32_t>(c);
+            } else {
+                cur = kLost;
+            }
+        }
+        slot->x = cur;
+        slot->y = j + 1;
+        return cur == kLost ? nullptr : pool + (static_cast<int64_t>(cur) * kCkChunk + j % kCkChunk) * kCk;
+    }
+};
+
+// The backward's view of one ray's records: the chunk of record j is found
+// by walking the links back from the ray's last chunk.
+struct PoolReader {
+    const double* pool;
+    const int32_t* link;
+    int32_t last, nrec;
+    __device__ const double* get(int j) const {
+        int32_t c = last;
+        for (int k = (nrec - 1) / kCkChunk; k > j / kCkChunk; --k) c = link[c];
+        return pool + (static_cast<int64_t>(c) * kCkChunk + j % kCkChunk) * kCk;
+    }
+};
+
 struct AdjArgs {
     int64_t n_rays;
     // forward
     uint32_t* counter;
     const int32_t* qmap;      // queue slot -> ray index
-    double* ck;               // n_rays * cap * kCk
+    double* pool;             // checkpoint chunks: pool_cap x kCkChunk x kCk doubles
+    int32_t* link;            // pool_cap chunk links
+    uint32_t* pool_counter;   // chunks taken (may exceed pool_cap: then the pool is too small)
+    uint32_t pool_cap;
+    int32_t* ck_last;         // per ray: its last chunk (-1: records lost)
+    int32_t* ck_nrec;         // per ray: records written
     uint8_t* cls;
     double* y;                // 8 x n_rays SoA (layout of bhx_rays.y)
     int32_t* steps;
@@ -79,9 +137,11 @@ k_adj_forward(const AdjConst* __restrict__ C, const AdjArgs P) {
     s.jet_wait = 0;
     s.pt = s.pph = s.jet = 0.0;
     bool live = false, done = false, out_wait = false;
-    double I_disc = 0.0;
-    int64_t ray = 0;
-    const int64_t slab = static_cast<int64_t>(F.cap) * kCk;
+    // per lane in LDS: checkpoint chunk, record count, ray index; the disc
+    // intensity of a ray that ended, until its last jet sample is in
+    __shared__ int4 lane_ray[256];
+    __shared__ double lane_idisc[256];
+    PoolWriter wr{P.pool, P.link, P.pool_counter, P.pool_cap, &lane_ray[threadIdx.x]};
     for (;;) {
         unsigned long long owners = __ballot(s.jet_wait != 0);
         while (owners != 0ull) {
@@ -111,7 +171,8 @@ k_adj_forward(const AdjConst* __restrict__ C, const AdjArgs P) {
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
         if (out_wait) {   // the ray ended on the last step
-            const double I = I_disc + F.kappa * s.jet;
+            const int64_t ray = lane_ray[threadIdx.x].z;
+            const double I = lane_idisc[threadIdx.x] + F.kappa * s.jet;
             P.I[ray] = I;
             P.key[ray] = (I > 0.0) ? max(1, min(s.nsteps, 32767)) : 0;   // 15-bit key, >= 1 for a lit ray
             if (I > 0.0) atomicAdd(P.n_lit, 1u);
@@ -122,22 +183,27 @@ k_adj_forward(const AdjConst* __restrict__ C, const AdjArgs P) {
             if ((int64_t)r >= P.n_rays) {
                 done = true;
             } else {
-                ray = P.qmap[r];
-                fwd_init(F, G, ray, s, P.ck + ray * slab);
+                const int32_t ray = P.qmap[r];
+                wr.start(ray);
+                fwd_init(F, G, ray, s, wr);
                 live = true;
             }
         }
         if (__ballot(live) == 0ull) break;
         if (!live) continue;
         double yo[6], se = 0.0;
-        const int term = fwd_step(F, s, P.ck + ray * slab, yo, &se, my_pend);
+        const int term = fwd_step(F, s, wr, yo, &se, my_pend);
         if (term >= 0) {
             const int64_t N = P.n_rays;
+            const int4 ck = *wr.slot;
+            const int64_t ray = ck.z;
+            P.ck_last[ray] = ck.x == kLost ? -1 : ck.x;
+            P.ck_nrec[ray] = ck.y;
             P.cls[ray] = (uint8_t)term;
             P.y[ray] = yo[0]; P.y[N + ray] = yo[1]; P.y[2 * N + ray] = yo[2]; P.y[3 * N + ray] = yo[3];
             P.y[4 * N + ray] = s.pt; P.y[5 * N + ray] = yo[4]; P.y[6 * N + ray] = yo[5]; P.y[7 * N + ray] = s.pph;
             P.steps[ray] = s.nsteps;
-            I_disc = (term == BHX_DISC) ? disc_intensity(F, yo[1], s.pt, s.pph) : 0.0;
+            lane_idisc[threadIdx.x] = (term == BHX_DISC) ? disc_intensity(F, yo[1], s.pt, s.pph) : 0.0;
             out_wait = true;
             live = false;
         }
@@ -156,7 +222,6 @@ __global__ void __launch_bounds__(256) k_adj_backward(const AdjConst* __restrict
     const RaygenArgs& G = C->G;
     const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nl = (int64_t)gridDim.x * blockDim.x;
-    const int64_t slab = static_cast<int64_t>(F.cap) * kCk;
     double g[4] = {0.0, 0.0, 0.0, 0.0};
     unsigned int mism = 0;
     const int64_t n_order = *P.n_lit;
@@ -164,7 +229,8 @@ __global__ void __launch_bounds__(256) k_adj_backward(const AdjConst* __restrict
         const int64_t ray = P.order[i];
         const double w = P.w[ray];
         if (w == 0.0) continue;
-        mism += bwd_ray(F, G, ray, P.steps[ray], P.cls[ray], w, P.ck + ray * slab, P.seg + lane, P.seg_stride, g,
+        const PoolReader rd{P.pool, P.link, P.ck_last[ray], P.ck_nrec[ray]};
+        mism += bwd_ray(F, G, ray, P.steps[ray], P.cls[ray], w, rd, P.seg + lane, P.seg_stride, g,
                         F.has_jet ? jet_lds[threadIdx.x / 64] : nullptr);
     }
     // block sums in a fixed order
@@ -191,7 +257,9 @@ struct bhx_adjoint {
     int K = 0;
     int64_t n_rays = 0;
     AdjConst host{};
-    DevBuf cst, ck, cls, y, steps, I, key, key_sorted, iota, order, sort_tmp, seg, partial, counters, qmap;
+    DevBuf cst, pool, link, ck_last, ck_nrec, cls, y, steps, I, key, key_sorted, iota, order, sort_tmp, seg,
+        partial, counters, qmap;
+    uint32_t pool_cap = 0;     // chunks in the checkpoint pool
     int grid_f = 0, grid_b = 0;
     double fwd_ms = 0.0, bwd_ms = 0.0;
     bool fwd_pending = false;
@@ -207,8 +275,7 @@ int adj_set_scene(bhx_adjoint* A, const bhx_scene* s) {
     const int S = std::max(1, s->sqrt_spp);
     const int64_t n = static_cast<int64_t>(s->width) * s->height * S * S;
     if (n != A->n_rays) return bhx_fail(ctx, BHX_E_INVALID, "scene size differs from the adjoint's");
-    if (s->max_steps / A->K + 1 != A->host.F.cap && A->host.F.cap != 0)
-        return bhx_fail(ctx, BHX_E_INVALID, "max_steps differs from the adjoint's");
+    if (s->max_steps < 1) return bhx_fail(ctx, BHX_E_INVALID, "max_steps must be > 0");
     AdjConst c;
     if (!make_frame(*s, A->K, c.F, c.G))
         return bhx_fail(ctx, BHX_E_INVALID,
@@ -221,6 +288,84 @@ int adj_set_scene(bhx_adjoint* A, const bhx_scene* s) {
     HIPCHK(hipMemcpy(A->cst.p, &c, sizeof c, hipMemcpyHostToDevice));
     A->host = c;
     A->scene = *s;
+    return BHX_OK;
+}
+
+// Recording forward on stream st (I: the caller's intensity buffer, or NULL
+// for the adjoint's own).
+int adj_record(bhx_adjoint* A, double* I, hipStream_t st) {
+    bhx_context* ctx = A->ctx;
+    AdjArgs P{};
+    P.n_rays = A->n_rays;
+    P.counter = static_cast<uint32_t*>(A->counters.p);
+    P.qmap = static_cast<const int32_t*>(A->qmap.p);
+    P.pool = static_cast<double*>(A->pool.p);
+    P.link = static_cast<int32_t*>(A->link.p);
+    P.pool_counter = reinterpret_cast<uint32_t*>(static_cast<char*>(A->counters.p) + 16);
+    P.pool_cap = A->pool_cap;
+    P.ck_last = static_cast<int32_t*>(A->ck_last.p);
+    P.ck_nrec = static_cast<int32_t*>(A->ck_nrec.p);
+    P.cls = static_cast<uint8_t*>(A->cls.p);
+    P.y = static_cast<double*>(A->y.p);
+    P.steps = static_cast<int32_t*>(A->steps.p);
+    P.I = I ? I : static_cast<double*>(A->I.p);
+    P.key = static_cast<int32_t*>(A->key.p);
+    P.n_lit = reinterpret_cast<unsigned int*>(static_cast<char*>(A->counters.p) + 8);
+    // counters: [0] queue, [8] lit rays, [16] pool chunks taken, [32] replay mismatches
+    HIPCHK(launch_zero_u64(static_cast<unsigned long long*>(A->counters.p), 8, st));
+    HIPCHK(hipEventRecord(A->ev[0], st));
+    hipLaunchKernelGGL(k_adj_forward, dim3(A->grid_f), dim3(256), 0, st,
+                       static_cast<const AdjConst*>(A->cst.p), P);
+    HIPCHK(hipGetLastError());
+    // the backward's ray order: lit rays first, most accepted steps first
+    // (a stable radix sort on the 15-bit step count: deterministic)
+    size_t tmp = A->sort_tmp.bytes;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairsDescending(A->sort_tmp.p, tmp, static_cast<const int32_t*>(A->key.p),
+                                                        static_cast<int32_t*>(A->key_sorted.p),
+                                                        static_cast<const int32_t*>(A->iota.p),
+                                                        static_cast<int32_t*>(A->order.p), A->n_rays, 0, 15, st));
+    HIPCHK(hipEventRecord(A->ev[1], st));
+    A->fwd_pending = true;
+    return BHX_OK;
+}
+
+// The adjoint sweep over the last forward's lit rays (synchronises st).
+int adj_sweep(bhx_adjoint* A, const double* w, double* grad, hipStream_t st) {
+    bhx_context* ctx = A->ctx;
+    AdjArgs P{};
+    P.n_rays = A->n_rays;
+    P.pool = static_cast<double*>(A->pool.p);
+    P.link = static_cast<int32_t*>(A->link.p);
+    P.ck_last = static_cast<int32_t*>(A->ck_last.p);
+    P.ck_nrec = static_cast<int32_t*>(A->ck_nrec.p);
+    P.cls = static_cast<uint8_t*>(A->cls.p);
+    P.steps = static_cast<int32_t*>(A->steps.p);
+    P.w = w;
+    P.order = static_cast<const int32_t*>(A->order.p);
+    P.seg = static_cast<double*>(A->seg.p);
+    P.seg_stride = static_cast<int64_t>(A->grid_b) * 256;
+    P.partial = static_cast<double*>(A->partial.p);
+    P.mismatch = reinterpret_cast<unsigned int*>(static_cast<char*>(A->counters.p) + 32);
+    P.n_lit = reinterpret_cast<unsigned int*>(static_cast<char*>(A->counters.p) + 8);
+    HIPCHK(launch_zero_u32(P.mismatch, 1, st));
+    HIPCHK(hipEventRecord(A->ev[2], st));
+    hipLaunchKernelGGL(k_adj_backward, dim3(A->grid_b), dim3(256), 0, st,
+                       static_cast<const AdjConst*>(A->cst.p), P);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(A->ev[3], st));
+    std::vector<double> part(4 * static_cast<size_t>(A->grid_b));
+    unsigned int mism = 0;
+    HIPCHK(hipMemcpyAsync(part.data(), A->partial.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&mism, P.mismatch, sizeof mism, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int c = 0; c < 4; ++c) grad[c] = 0.0;
+    for (int b = 0; b < A->grid_b; ++b)
+        for (int c = 0; c < 4; ++c) grad[c] += part[4 * b + c];
+    A->mismatches = mism;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, A->ev[2], A->ev[3]) == hipSuccess) A->bwd_ms = ms;
+    if (A->fwd_pending && hipEventElapsedTime(&ms, A->ev[0], A->ev[1]) == hipSuccess) A->fwd_ms = ms;
+    A->fwd_pending = false;
     return BHX_OK;
 }
 
@@ -238,10 +383,11 @@ int bhx_adjoint_create(bhx_context* ctx, const bhx_scene* s, int32_t K, bhx_adjo
     const int S = std::max(1, s->sqrt_spp);
     const int64_t n = static_cast<int64_t>(s->width) * s->height * S * S;
     if (n > 2000000000LL) return bhx_fail(ctx, BHX_E_TOO_LARGE, "more than 2e9 samples");
-    const int cap = s->max_steps / K + 1;
-    const double ck_bytes = static_cast<double>(n) * cap * kCk * sizeof(double);
-    if (ck_bytes > 64.0 * (1ull << 30))
-        return bhx_fail(ctx, BHX_E_TOO_LARGE, "checkpoint slab above 64 GiB: raise K or lower max_steps");
+    // first guess of the checkpoint pool: rays of up to 512 accepted steps
+    // (a frame at rtol 1e-10 averages ~300); a forward that finds it short
+    // tells the first backward after it, which grows the pool to the need
+    const int64_t per_ray = (512 + static_cast<int64_t>(K) * kCkChunk - 1) / (static_cast<int64_t>(K) * kCkChunk) + 1;
+    const int64_t chunks = std::min<int64_t>(n * per_ray, kMaxPoolChunks);
     HIPCHK(hipSetDevice(ctx->device));
     bhx_adjoint* A = new bhx_adjoint();
     A->ctx = ctx;
@@ -259,7 +405,11 @@ int bhx_adjoint_create(bhx_context* ctx, const bhx_scene* s, int32_t K, bhx_adjo
         per_cu = 1;
     A->grid_b = per_cu * ctx->n_cu;
     const int64_t lanes = static_cast<int64_t>(A->grid_b) * 256;
-    if (A->cst.ensure(sizeof(AdjConst)) != hipSuccess || A->ck.ensure(static_cast<size_t>(ck_bytes)) != hipSuccess ||
+    A->pool_cap = static_cast<uint32_t>(chunks);
+    if (A->cst.ensure(sizeof(AdjConst)) != hipSuccess ||
+        A->pool.ensure(sizeof(double) * kCk * kCkChunk * static_cast<size_t>(chunks)) != hipSuccess ||
+        A->link.ensure(sizeof(int32_t) * static_cast<size_t>(chunks)) != hipSuccess ||
+        A->ck_last.ensure(sizeof(int32_t) * n) != hipSuccess || A->ck_nrec.ensure(sizeof(int32_t) * n) != hipSuccess ||
         A->cls.ensure(n) != hipSuccess || A->y.ensure(sizeof(double) * 8 * n) != hipSuccess ||
         A->steps.ensure(sizeof(int32_t) * n) != hipSuccess || A->I.ensure(sizeof(double) * n) != hipSuccess ||
         A->order.ensure(sizeof(int32_t) * n) != hipSuccess || A->key.ensure(sizeof(int32_t) * n) != hipSuccess ||
@@ -312,7 +462,7 @@ int bhx_adjoint_destroy(bhx_adjoint* A) {
     if (!A) return BHX_OK;
     (void)hipSetDevice(A->ctx->device);
     (void)hipDeviceSynchronize();
-    for (DevBuf* d : {&A->cst, &A->ck, &A->cls, &A->y, &A->steps, &A->I, &A->key, &A->key_sorted, &A->iota,
+    for (DevBuf* d : {&A->cst, &A->pool, &A->link, &A->ck_last, &A->ck_nrec, &A->cls, &A->y, &A->steps, &A->I, &A->key, &A->key_sorted, &A->iota,
                       &A->order, &A->sort_tmp, &A->seg, &A->partial, &A->counters, &A->qmap})
         d->release();
     for (auto& e : A->ev)
@@ -332,34 +482,7 @@ int bhx_adjoint_forward(bhx_adjoint* A, double* I, void* stream) {
     if (!A) return BHX_E_INVALID;
     bhx_context* ctx = A->ctx;
     HIPCHK(hipSetDevice(ctx->device));
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    AdjArgs P{};
-    P.n_rays = A->n_rays;
-    P.counter = static_cast<uint32_t*>(A->counters.p);
-    P.qmap = static_cast<const int32_t*>(A->qmap.p);
-    P.ck = static_cast<double*>(A->ck.p);
-    P.cls = static_cast<uint8_t*>(A->cls.p);
-    P.y = static_cast<double*>(A->y.p);
-    P.steps = static_cast<int32_t*>(A->steps.p);
-    P.I = I ? I : static_cast<double*>(A->I.p);
-    P.key = static_cast<int32_t*>(A->key.p);
-    P.n_lit = reinterpret_cast<unsigned int*>(static_cast<char*>(A->counters.p) + 8);
-    // counters: [0] queue, [8] lit rays, [32] replay mismatches
-    HIPCHK(launch_zero_u64(static_cast<unsigned long long*>(A->counters.p), 8, st));
-    HIPCHK(hipEventRecord(A->ev[0], st));
-    hipLaunchKernelGGL(k_adj_forward, dim3(A->grid_f), dim3(256), 0, st,
-                       static_cast<const AdjConst*>(A->cst.p), P);
-    HIPCHK(hipGetLastError());
-    // the backward's ray order: lit rays first, most accepted steps first
-    // (a stable radix sort on the 15-bit step count: deterministic)
-    size_t tmp = A->sort_tmp.bytes;
-    HIPCHK(hipcub::DeviceRadixSort::SortPairsDescending(A->sort_tmp.p, tmp, static_cast<const int32_t*>(A->key.p),
-                                                        static_cast<int32_t*>(A->key_sorted.p),
-                                                        static_cast<const int32_t*>(A->iota.p),
-                                                        static_cast<int32_t*>(A->order.p), A->n_rays, 0, 15, st));
-    HIPCHK(hipEventRecord(A->ev[1], st));
-    A->fwd_pending = true;
-    return BHX_OK;
+    return adj_record(A, I, static_cast<hipStream_t>(stream));
 }
 
 int bhx_adjoint_backward(bhx_adjoint* A, const double* w, double* grad, void* stream) {
@@ -367,38 +490,24 @@ int bhx_adjoint_backward(bhx_adjoint* A, const double* w, double* grad, void* st
     bhx_context* ctx = A->ctx;
     HIPCHK(hipSetDevice(ctx->device));
     hipStream_t st = static_cast<hipStream_t>(stream);
-    AdjArgs P{};
-    P.n_rays = A->n_rays;
-    P.ck = static_cast<double*>(A->ck.p);
-    P.cls = static_cast<uint8_t*>(A->cls.p);
-    P.steps = static_cast<int32_t*>(A->steps.p);
-    P.w = w;
-    P.order = static_cast<const int32_t*>(A->order.p);
-    P.seg = static_cast<double*>(A->seg.p);
-    P.seg_stride = static_cast<int64_t>(A->grid_b) * 256;
-    P.partial = static_cast<double*>(A->partial.p);
-    P.mismatch = reinterpret_cast<unsigned int*>(static_cast<char*>(A->counters.p) + 32);
-    P.n_lit = reinterpret_cast<unsigned int*>(static_cast<char*>(A->counters.p) + 8);
-    HIPCHK(launch_zero_u32(P.mismatch, 1, st));
-    HIPCHK(hipEventRecord(A->ev[2], st));
-    hipLaunchKernelGGL(k_adj_backward, dim3(A->grid_b), dim3(256), 0, st,
-                       static_cast<const AdjConst*>(A->cst.p), P);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(A->ev[3], st));
-    std::vector<double> part(4 * static_cast<size_t>(A->grid_b));
-    unsigned int mism = 0;
-    HIPCHK(hipMemcpyAsync(part.data(), A->partial.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&mism, P.mismatch, sizeof mism, hipMemcpyDeviceToHost, st));
+    // a forward that ran out of pool chunks counted what it needed: grow the
+    // pool to that and record the same forward again (deterministic: the same
+    // rays, steps and intensities; the caller's intensity buffer is not
+    // written again)
+    uint32_t taken = 0;
+    HIPCHK(hipMemcpyAsync(&taken, static_cast<char*>(A->counters.p) + 16, sizeof taken, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    for (int c = 0; c < 4; ++c) grad[c] = 0.0;
-    for (int b = 0; b < A->grid_b; ++b)
-        for (int c = 0; c < 4; ++c) grad[c] += part[4 * b + c];
-    A->mismatches = mism;
-    float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, A->ev[2], A->ev[3]) == hipSuccess) A->bwd_ms = ms;
-    if (A->fwd_pending && hipEventElapsedTime(&ms, A->ev[0], A->ev[1]) == hipSuccess) A->fwd_ms = ms;
-    A->fwd_pending = false;
-    return BHX_OK;
+    if (taken > A->pool_cap) {
+        if (static_cast<int64_t>(taken) > kMaxPoolChunks)
+            return bhx_fail(ctx, BHX_E_TOO_LARGE, "checkpoint pool above 128 GiB: raise K");
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(A->pool.ensure(sizeof(double) * kCk * kCkChunk * static_cast<size_t>(taken)));
+        HIPCHK(A->link.ensure(sizeof(int32_t) * static_cast<size_t>(taken)));
+        A->pool_cap = taken;
+        const int rc = adj_record(A, nullptr, st);
+        if (rc) return rc;
+    }
+    return adj_sweep(A, w, grad, st);
 }
 
 int bhx_adjoint_rays(const bhx_adjoint* A, bhx_rays* rays) {
@@ -433,7 +542,7 @@ int bhx_adjoint_info(bhx_adjoint* A, int64_t* ck_bytes, int64_t* lit_rays, int64
     }
     unsigned int nd = 0;
     HIPCHK(hipMemcpy(&nd, static_cast<char*>(A->counters.p) + 8, sizeof nd, hipMemcpyDeviceToHost));
-    if (ck_bytes) *ck_bytes = static_cast<int64_t>(A->ck.bytes);
+    if (ck_bytes) *ck_bytes = static_cast<int64_t>(A->pool.bytes + A->link.bytes);
     if (lit_rays) *lit_rays = nd;
     if (mismatches) *mismatches = A->mismatches;
     if (fwd_ms) *fwd_ms = A->fwd_ms;

@@ -41,7 +41,7 @@ constexpr int kCk = 8;   // checkpoint record: y[6], h, controller memory (beta 
 struct Frame {
     // integrator (src/render.cpp:71-86)
     double a, a2, r_h, r_stop, rtol, atol, r_escape, h0, r_in, r_out;
-    int32_t max_steps, K, cap, sg;
+    int32_t max_steps, K, sg, pad_k;
     // Novikov-Thorne disc (src/disc.h:46-121) and the spin derivatives of its
     // spin-only constants
     double disc_a, flux_scale, dflux_da, eddington;
@@ -69,6 +69,16 @@ __host__ __device__ inline void write_ck(double* ck, const Ray& s) {
     ck[6] = s.h;
     ck[7] = s.lpw;
 }
+
+// A ray's checkpoints in one flat array: record j at base + j kCk.  The
+// forward and the backward take any store with put(j) (the slot to write
+// record j to, or NULL to drop it; records come in order) and get(j); the
+// kernels use a chunked pool (adjoint.hip).
+struct FlatCk {
+    double* base;
+    __host__ __device__ double* put(int j) { return base + static_cast<int64_t>(j) * kCk; }
+    __host__ __device__ const double* get(int j) const { return base + static_cast<int64_t>(j) * kCk; }
+};
 
 // One DP5 attempt from s (the tracer's arithmetic, src/geodesic.h:130-169).
 // u = err^2 (err the RMS scaled error; 1e20 when not finite): the square root
@@ -741,7 +751,8 @@ __host__ __device__ inline void ray_pixel(const RaygenArgs& G, int64_t ray, int&
 
 // The camera photon of a ray and the integrator's start (src/render.cpp:
 // 76-86); checkpoint 0 is written to ck.
-__host__ __device__ inline void fwd_init(const Frame& F, const RaygenArgs& G, int64_t ray, Ray& s, double* ck) {
+template <class Ck>
+__host__ __device__ inline void fwd_init(const Frame& F, const RaygenArgs& G, int64_t ray, Ray& s, Ck& ck) {
     int px, py, k;
     ray_pixel(G, ray, px, py, k);
     double d[3];
@@ -761,7 +772,7 @@ __host__ __device__ inline void fwd_init(const Frame& F, const RaygenArgs& G, in
     s.jet_wait = 0;
     s.nsteps = 0;
     s.tries = 0;
-    write_ck(ck, s);
+    if (double* r = ck.put(0)) write_ck(r, s);
 }
 
 // Dense-output coefficients of the components (r, theta, p_r, p_theta).
@@ -780,7 +791,8 @@ __host__ __device__ inline void dense4(const Ray& s, const Att& A, double cf[4][
 // With pend (a 17-double slot), a step's jet sample is not taken here: its
 // dense output (r, theta, p_r), s_end and h go to pend and s.jet_wait is set;
 // the caller adds j0 h s_end sum_j w_j jet_value_node(j) / kConeSub to s.jet.
-__host__ __device__ inline int fwd_step(const Frame& F, Ray& s, double* ck, double yo[6], double* s_hit,
+template <class Ck>
+__host__ __device__ inline int fwd_step(const Frame& F, Ray& s, Ck& ck, double yo[6], double* s_hit,
                                         double* pend = nullptr) {
     clamp_h(F, s);
     const RayK K = ray_k(F.a, F.a2, s.pt, s.pph);
@@ -839,7 +851,7 @@ __host__ __device__ inline int fwd_step(const Frame& F, Ray& s, double* ck, doub
                 for (int c = 0; c < 6; ++c) yo[c] = s.y[c];
                 term = BHX_EXHAUSTED;
             } else if (s.nsteps % F.K == 0) {
-                write_ck(ck + (s.nsteps / F.K) * kCk, s);
+                if (double* r = ck.put(s.nsteps / F.K)) write_ck(r, s);
             }
         }
         return term;
@@ -905,8 +917,9 @@ __host__ __device__ inline bool same_ck(const Ray& s, const double* ck) {
 // adds d(w I)/d(spin, theta_cam, eddington, r_cam) to grad.  Returns the
 // number of replayed segments whose end state differed from the next
 // checkpoint (0 when the replay reproduces the forward exactly).
+template <class Ck>
 __host__ __device__ inline int bwd_ray(const Frame& F, const RaygenArgs& G, int64_t ray, int n_acc, int cls,
-                                       double w, const double* ck, double* seg, int64_t stride, double grad[4],
+                                       double w, const Ck& ck, double* seg, int64_t stride, double grad[4],
                                        double* lds = nullptr) {
     int mismatch = 0;
     int px, py, k;
@@ -930,7 +943,7 @@ __host__ __device__ inline int bwd_ray(const Frame& F, const RaygenArgs& G, int6
     const int n_reg = disc ? n_acc - 1 : n_acc;
     const int j_last = (n_reg == 0) ? 0 : (n_reg - 1) / F.K;
     const int last_from = j_last * F.K;
-    replay(F, s, ck + j_last * kCk, last_from, n_reg, seg, stride);
+    replay(F, s, ck.get(j_last), last_from, n_reg, seg, stride);
     if (disc) {
         double r_hit = 0.0;
         bool jet_term = false;
@@ -971,8 +984,8 @@ __host__ __device__ inline int bwd_ray(const Frame& F, const RaygenArgs& G, int6
         const int n_from = j * F.K;
         const int n_to = (j == j_last) ? n_last : n_from + F.K;
         if (j != j_last) {
-            replay(F, s, ck + j * kCk, n_from, n_to, seg, stride);
-            if (!same_ck(s, ck + (j + 1) * kCk)) ++mismatch;
+            replay(F, s, ck.get(j), n_from, n_to, seg, stride);
+            if (!same_ck(s, ck.get(j + 1))) ++mismatch;
         }
         for (int i = n_to - n_from - 1; i >= 0; --i) {
             const double* o = seg + static_cast<int64_t>(i) * kSeg * stride;

@@ -44,7 +44,6 @@ inline bool make_frame(const bhx_scene& s, int K, Frame& F, RaygenArgs& G) {
     F.r_escape = T.r_escape; F.h0 = T.h0; F.r_in = T.r_in; F.r_out = T.r_out;
     F.max_steps = T.max_steps;
     F.K = K;
-    F.cap = T.max_steps / K + 1;
     F.sg = S.disc_sg;
     F.disc_a = S.disc_a;
     F.flux_scale = S.flux_scale;

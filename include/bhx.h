@@ -317,8 +317,12 @@ int bhx_batch_autotune(bhx_batch *batch, int32_t reps, bhx_launch *best, double 
  * intensity at nu0 integrated along the ray - and its discrete adjoint: given
  * dL/dI per sample, the gradient of L with respect to (spin, cam_theta [rad],
  * eddington, cam_r).  The forward records a checkpoint of the integrator
- * state every K accepted steps (HBM: samples x (max_steps / K + 1) x 64
- * bytes); the backward replays each K-step segment from its checkpoint with
+ * state every K accepted steps (64 bytes each) into a device pool of
+ * 8-record chunks that each ray takes as it goes, so the pool holds about
+ * sum over rays of (steps / K + 1) x 64 bytes, whatever max_steps is; a
+ * forward that finds the pool short still completes its image, and the next
+ * backward grows the pool to the forward's count and records the same
+ * forward again before it sweeps; the backward replays each K-step segment from its checkpoint with
  * the same arithmetic and sweeps it in reverse with the step sizes frozen,
  * and differentiates the disc crossing time by the implicit-function
  * theorem.  No counterpart in the reference, which has no derivatives; the
@@ -330,20 +334,22 @@ typedef struct bhx_adjoint bhx_adjoint;
 
 int bhx_adjoint_create(bhx_context *ctx, const bhx_scene *scene, int32_t K, bhx_adjoint **out);
 int bhx_adjoint_destroy(bhx_adjoint *adj);
-/* New parameters for the same frame size and max_steps (synchronises). */
+/* New parameters for a frame of the same size (synchronises). */
 int bhx_adjoint_set_scene(bhx_adjoint *adj, const bhx_scene *scene);
 /* Recording forward: I (device, one double per sample) and the checkpoints,
  * then the backward's ray order (lit rays, I > 0, by step count, a device
  * radix sort).  Stream-ordered, no host synchronisation. */
 int bhx_adjoint_forward(bhx_adjoint *adj, double *I, void *stream);
 /* w (device, one double per sample) = dL/dI; grad (host, 4 doubles) =
- * dL/d(spin, cam_theta, eddington, cam_r).  Synchronous. */
+ * dL/d(spin, cam_theta, eddington, cam_r) at the last forward's parameters.
+ * Synchronous.  BHX_E_TOO_LARGE when the checkpoints would need more than
+ * 128 GiB (raise K). */
 int bhx_adjoint_backward(bhx_adjoint *adj, const double *w, double *grad, void *stream);
 /* Device views of the last forward: class, hit state (8 x n SoA), steps. */
 int bhx_adjoint_rays(const bhx_adjoint *adj, bhx_rays *rays);
 /* The same copied to HOST buffers (any may be NULL; synchronises `stream`). */
 int bhx_adjoint_copy_rays(bhx_adjoint *adj, uint8_t *cls, double *y, int32_t *steps, void *stream);
-/* Checkpoint slab size, lit rays (I > 0) of the last forward, replayed segments
+/* Checkpoint pool size in bytes, lit rays (I > 0) of the last forward, replayed segments
  * that did not reproduce their checkpoint (0 expected), kernel times. */
 int bhx_adjoint_info(bhx_adjoint *adj, int64_t *ck_bytes, int64_t *lit_rays, int64_t *mismatches,
                      double *fwd_ms, double *bwd_ms);

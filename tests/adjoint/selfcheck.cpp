@@ -195,12 +195,13 @@ static double ray_I(const bhx_scene& sc, int K, int64_t ray, int* cls, int* n_ac
     Frame F;
     RaygenArgs G;
     make_frame(sc, K, F, G);
-    std::vector<double> ck(static_cast<size_t>(F.cap) * kCk);
+    std::vector<double> ck(static_cast<size_t>(F.max_steps / K + 1) * kCk);
+    FlatCk store{ck.data()};
     Ray s;
-    fwd_init(F, G, ray, s, ck.data());
+    fwd_init(F, G, ray, s, store);
     double yo[6], se = 0.0;
     int term = -1;
-    while (term < 0) term = fwd_step(F, s, ck.data(), yo, &se);
+    while (term < 0) term = fwd_step(F, s, store, yo, &se);
     *cls = term;
     *n_acc = s.nsteps;
     return (term == BHX_DISC ? disc_intensity(F, yo[1], s.pt, s.pph) : 0.0) + F.kappa * s.jet;
@@ -217,19 +218,20 @@ static void check_ray(const char* name, const bhx_scene& sc, int K, int64_t stri
         std::printf("%s 0 1 mismatch 0\n", name);
         return;
     }
-    std::vector<double> ck(static_cast<size_t>(F.cap) * kCk), seg(static_cast<size_t>(K + 1) * kSeg);
+    std::vector<double> ck(static_cast<size_t>(F.max_steps / K + 1) * kCk), seg(static_cast<size_t>(K + 1) * kSeg);
+    FlatCk store{ck.data()};
     double worst = 0.0;
     int n = 0, mism = 0, n_jet = 0;
     for (int64_t ray = 0; ray < (int64_t)sc.width * sc.height && n < n_max; ray += stride) {
         Ray s;
-        fwd_init(F, G, ray, s, ck.data());
+        fwd_init(F, G, ray, s, store);
         double yo[6], se = 0.0;
         int term = -1;
-        while (term < 0) term = fwd_step(F, s, ck.data(), yo, &se);
+        while (term < 0) term = fwd_step(F, s, store, yo, &se);
         const double I0 = (term == BHX_DISC ? disc_intensity(F, yo[1], s.pt, s.pph) : 0.0) + F.kappa * s.jet;
         if (!(I0 > 0.0)) continue;
         double grad[4] = {0, 0, 0, 0};
-        mism += bwd_ray(F, G, ray, s.nsteps, term, 1.0, ck.data(), seg.data(), 1, grad);
+        mism += bwd_ray(F, G, ray, s.nsteps, term, 1.0, store, seg.data(), 1, grad);
         double fd[4];
         bool ok = true;
         for (int p = 0; p < 4; ++p) {

@@ -153,3 +153,53 @@ def test_autograd_backward_after_a_later_forward(tracer):
     assert np.array_equal(p1.grad.numpy(), g1)
     assert np.array_equal(p2.grad.numpy(), g2)
     dr.close()
+
+
+def test_checkpoint_pool_grows_to_the_frame(tracer):
+    """At rtol 1e-12 the rays take ~3x the steps the first guess of the
+    checkpoint pool allows for: the first backward finds the forward short,
+    grows the pool to the count the forward made and records it again; the
+    gradient is the one of a forward and backward with the grown pool, bit
+    for bit, and every replay reproduces its checkpoint."""
+    import torch
+    sc = S.disc_frame(40, 40, rtol=1e-12, atol=1e-14)
+    dr = DiscRender(tracer, sc, K=16)
+    I = dr.forward_raw(P0)
+    before = dr.info()["ck_bytes"]
+    w = torch.linspace(-1.0, 1.0, sc.n_rays, dtype=torch.float64, device=I.device)
+    g1 = dr.backward_raw(w)
+    info = dr.info()
+    steps = dr.rays()["steps"]
+    print(f"[pool] mean steps {steps.mean():.0f}, pool {before} -> {info['ck_bytes']} bytes", flush=True)
+    assert info["ck_bytes"] > before and info["mismatches"] == 0
+    I2 = dr.forward_raw(P0)
+    g2 = dr.backward_raw(w)
+    assert torch.equal(I, I2) and np.array_equal(g1, g2)
+    assert dr.info()["mismatches"] == 0
+    # the pool holds what the rays need: ceil(records / 8) chunks of 8 records
+    # of 64 B plus a 4-byte link each; a ray of N accepted steps wrote
+    # floor(N / K) + 1 records, or one fewer when it ended on a multiple of K
+    taken = info["ck_bytes"] // (8 * 64 + 4)
+    assert info["ck_bytes"] == taken * (8 * 64 + 4)
+    lo = int(np.sum(((np.maximum(steps - 1, 0) // 16 + 1) + 7) // 8))
+    hi = int(np.sum(((steps // 16 + 1) + 7) // 8))
+    assert lo <= taken <= hi, (lo, taken, hi)
+    dr.close()
+
+
+def test_full_frame_at_K16(tracer):
+    """The 1024^2 differentiation frame at K = 16 (a fixed max_steps / K + 1
+    slab per ray would be 84 GB): the pool is sized by the rays' steps."""
+    import torch
+    sc = S.disc_frame(1024, 1024)
+    dr = DiscRender(tracer, sc, K=16)
+    I = dr.forward_raw(P0)
+    w = torch.ones(sc.n_rays, dtype=torch.float64, device=I.device)
+    g = dr.backward_raw(w)
+    info = dr.info()
+    steps = dr.rays()["steps"]
+    print(f"[1024 K16] pool {info['ck_bytes'] / 1e9:.2f} GB, lit {info['lit_rays']}, mean steps "
+          f"{steps.mean():.0f}, grad {g}", flush=True)
+    assert info["mismatches"] == 0 and np.all(np.isfinite(g))
+    assert info["ck_bytes"] < 8e9
+    dr.close()
