@@ -230,13 +230,14 @@ def batched_leg(tr, torch, rounds=3):
     }
 
 
-def adjoint_leg(tr, torch, width=512, Ks=(16, 64, 256), rounds=3, fd=True, threads=16):
+def adjoint_leg(tr, torch, width=512, Ks=(16, 32, 64, 128, 256), rounds=3, fd=True, threads=16):
     """Differentiable disc image at width^2 (a=0.7, i=45 deg, 0.1 Edd, r=500M;
     target a=0.9, i=50 deg): loss = sum (G*I - G*I_target)^2 with a Gaussian
-    sigma = 2 px.  Times a forward render (the plain path: trace + shade) and
-    a gradient pass (recording forward + adjoint) for each checkpoint interval
-    K; optionally checks the gradient against central differences (h = 1e-6
-    relative) of the oracle's images."""
+    sigma = 2 px.  Times a forward render (the plain path: trace + shade); the
+    library's autotuner picks the checkpoint interval K among Ks, and a
+    gradient pass (recording forward + adjoint) is timed at that K and at each
+    candidate; optionally checks the gradient against central differences
+    (h = 1e-6 relative) of the oracle's images."""
     import math
     import numpy as np
     from blackhole_amd import scene as S
@@ -261,17 +262,19 @@ def adjoint_leg(tr, torch, width=512, Ks=(16, 64, 256), rounds=3, fd=True, threa
     res = {"workload": f"{width}x{width} disc image, a=0.7, i=45 deg, 0.1 Edd, r_obs=500M; target "
                        f"a=0.9, i=50 deg; L = sum (G*I - G*I_t)^2, Gaussian sigma 2 px",
            "forward_render_ms": round(fwd_ms, 3), "per_K": {}}
-    best = None
-    w = None
-    for K in Ks:
-        dr = DiscRender(tr, sc, K=K)
-        if w is None:
-            It = dr.forward_raw(PT).view(width, width).clone()
-            I0 = dr.forward_raw(P0).view(width, width).clone().requires_grad_(True)
-            L = ((gaussian_blur(I0) - gaussian_blur(It)) ** 2).sum()
-            L.backward()
-            w = I0.grad.reshape(-1).contiguous()
-            loss = float(L)
+    dr = DiscRender(tr, sc, K=64)
+    It = dr.forward_raw(PT).view(width, width).clone()
+    I0 = dr.forward_raw(P0).view(width, width).clone().requires_grad_(True)
+    L = ((gaussian_blur(I0) - gaussian_blur(It)) ** 2).sum()
+    L.backward()
+    w = I0.grad.reshape(-1).contiguous()
+    loss = float(L)
+    dr.forward_raw(P0)
+    tuned_K, tuned_ms = dr.autotune(w, Ks=Ks, reps=2)
+    res["autotuned_K"] = tuned_K
+    res["autotune_ms_per_pass"] = round(tuned_ms, 3)
+
+    def grad_pass():
         dr.forward_raw(P0)
         g = dr.backward_raw(w)
         torch.cuda.synchronize()
@@ -280,15 +283,21 @@ def adjoint_leg(tr, torch, width=512, Ks=(16, 64, 256), rounds=3, fd=True, threa
             dr.forward_raw(P0)
             g = dr.backward_raw(w)
         torch.cuda.synchronize()
-        ms = (time.perf_counter() - t0) / rounds * 1e3
+        return (time.perf_counter() - t0) / rounds * 1e3, g
+
+    for K in Ks:   # every candidate, for the record
+        dr.autotune(w, Ks=(K,), reps=1)
+        ms, g = grad_pass()
         info = dr.info()
         res["per_K"][str(K)] = {"grad_pass_ms": round(ms, 3), "record_forward_ms": round(info["fwd_ms"], 3),
                                 "adjoint_ms": round(info["bwd_ms"], 3), "checkpoint_bytes": info["ck_bytes"],
                                 "replay_mismatches": info["mismatches"], "grad_over_fwd": round(ms / fwd_ms, 3)}
-        if best is None or ms < best[1]:
-            best = (K, ms, g)
-        dr.close()
+    dr.autotune(w, Ks=(tuned_K,), reps=1)
+    ms, g = grad_pass()
+    best = (tuned_K, ms, g)
+    dr.close()
     res["best_K"] = best[0]
+    res["grad_pass_ms"] = round(best[1], 3)
     res["grad_over_fwd"] = round(best[1] / fwd_ms, 3)
     res["loss"] = loss
     res["grad"] = dict(zip(("spin", "cam_theta", "eddington", "cam_r"), (float(v) for v in best[2])))
@@ -337,13 +346,13 @@ def sweep_leg(tr, torch, n_frames=120, width=1024, K=64, rounds=2, grad=True, fd
     (r0/r)^2 to 500 M, 250k stars, camera at 2500 M) as ONE batched
     persistent launch replayed as a hipGraph; then dL/dtheta of every frame
     from the discrete adjoint, L_k = sum (G*I_k - G*I_(k-1))^2 (G a Gaussian
-    of sigma 2 px, I the disc + jet intensity, I_(-1) = 0), one recording
-    forward + adjoint per frame.  A few frames' dL/dtheta are checked against
-    central differences (h = 1e-6 rad) of the same forward over the pixels
-    whose class holds."""
+    of sigma 2 px, I the disc + jet intensity, I_(-1) = 0), from one batched
+    recording forward over all frames and one batched adjoint.  A few frames'
+    dL/dtheta are checked against central differences (h = 1e-6 rad) of the
+    same forward over the pixels whose class holds."""
     import numpy as np
     from blackhole_amd import Batch, scene as S
-    from blackhole_amd.diff import DiscRender, gaussian_blur
+    from blackhole_amd.diff import DiscBatch, DiscRender, gaussian_blur
     frames = S.quasar_sweep(n_frames, width, width)
     n_rays = sum(sc.n_rays for sc in frames)
     res = {"workload": f"{n_frames} frames {width}x{width}: a=0.998, inclination 0.5->90 deg, NT disc "
@@ -367,48 +376,66 @@ def sweep_leg(tr, torch, n_frames=120, width=1024, K=64, rounds=2, grad=True, fd
                 "jet_rays": int(sum(st.jet_rays for st in stats))})
     if not grad:
         return res
-    dr = DiscRender(tr, frames[0], K=K)
-    prev = None
-    dth, per_ms, w_keep = [], [], {}
+    # every frame's gradient from one batched adjoint: one recording forward
+    # over all frames' rays, dL_k/dI_k per frame, one backward
+    db = DiscBatch(tr, frames, K=K)
     torch.cuda.synchronize()
     t_all = time.perf_counter()
-    for k, sc in enumerate(frames):
-        t0 = time.perf_counter()
-        p = (sc.spin, sc.cam_theta, sc.eddington, sc.cam_r)
-        I = dr.forward_raw(p).view(width, width).clone().requires_grad_(True)
-        T = prev if prev is not None else torch.zeros_like(I)
-        L = ((gaussian_blur(I) - gaussian_blur(T)) ** 2).sum()
+    I = db.forward_raw()
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter()
+    w = torch.empty_like(I)
+    prev = torch.zeros(width, width, dtype=torch.float64, device=I.device)
+    for k in range(n_frames):
+        Ik = I[k].view(width, width).clone().requires_grad_(True)
+        L = ((gaussian_blur(Ik) - gaussian_blur(prev)) ** 2).sum()
         L.backward()
-        w = I.grad.reshape(-1).contiguous()
-        g = dr.backward_raw(w)
-        per_ms.append((time.perf_counter() - t0) * 1e3)
-        dth.append(float(g[1]))
-        if k in fd_frames:
-            w_keep[k] = (w, p, T)
-        prev = I.detach()
+        w[k] = Ik.grad.reshape(-1)
+        prev = I[k].view(width, width)
+    torch.cuda.synchronize()
+    t_w = time.perf_counter()
+    g = db.backward_raw(w)
     grad_s = time.perf_counter() - t_all
-    info = dr.info()
-    res.update({"grad_K": K, "grad_total_s": round(grad_s, 3), "grad_frame_ms_mean": round(float(np.mean(per_ms)), 2),
-                "grad_frame_ms_max": round(float(np.max(per_ms)), 2), "checkpoint_bytes": info["ck_bytes"],
-                "dL_dtheta": [float(f"{v:.6g}") for v in dth]})
-    checks = []
-    for k, (w, p, T) in sorted(w_keep.items()):
+    info = db.info()
+    res.update({"grad_K": K, "grad_total_s": round(grad_s, 3),
+                "grad_forward_s": round(t_fwd - t_all, 3), "grad_loss_weights_s": round(t_w - t_fwd, 3),
+                "grad_backward_s": round(time.perf_counter() - t_w, 3),
+                "grad_frame_ms_mean": round(grad_s * 1e3 / n_frames, 2),
+                "grad_mrays_s": round(n_rays / grad_s / 1e6, 3),
+                "checkpoint_bytes": info["ck_bytes"], "replay_mismatches": info["mismatches"],
+                "dL_dtheta": [float(f"{v:.6g}") for v in g[:, 1]]})
+    # a few frames against central differences of the same forward over the
+    # pixels whose class holds: the batch's backward with dL/dI zeroed
+    # outside them (and outside the checked frames), and one frame through
+    # the single-frame adjoint for comparison
+    dr = DiscRender(tr, frames[0], K=K)
+    wm_all = torch.zeros_like(w)
+    checks = {}
+    for k in fd_frames:
+        sc = frames[k]
+        p = (sc.spin, sc.cam_theta, sc.eddington, sc.cam_r)
         h = 1e-6
         masks, Is = [], []
         for sgn in (0, 1, -1):
             q = (p[0], p[1] + sgn * h, p[2], p[3])
             Is.append(dr.forward_raw(q).clone())
             masks.append(dr.rays()["cls"])
-        stable = (masks[1] == masks[0]) & (masks[2] == masks[0])
-        wm = torch.from_numpy(np.where(stable, w.cpu().numpy(), 0.0)).to(w.device)
-        fd = float(torch.sum(wm * (Is[1] - Is[2])) / (2 * h))
+        stable = torch.from_numpy((masks[1] == masks[0]) & (masks[2] == masks[0])).to(w.device)
+        wm_all[k] = torch.where(stable, w[k], torch.zeros_like(w[k]))
+        fd = float(torch.sum(wm_all[k] * (Is[1] - Is[2])) / (2 * h))
         dr.forward_raw(p)
-        ga = float(dr.backward_raw(wm)[1])
-        checks.append({"frame": k, "incl_deg": round(float(np.degrees(p[1])), 3), "adjoint": ga, "fd": fd,
-                       "rel_err": abs(ga - fd) / abs(fd) if fd else None,
-                       "stable_pixels": int(stable.sum())})
-    res["fd_check_theta"] = checks
+        single = float(dr.backward_raw(wm_all[k])[1])
+        checks[k] = {"frame": k, "incl_deg": round(float(np.degrees(p[1])), 3), "fd": fd, "single_frame": single,
+                     "stable_pixels": int(stable.sum())}
+    gm = db.backward_raw(wm_all)
+    for k, c in checks.items():
+        c["batch"] = float(gm[k, 1])
+        c["rel_err"] = abs(c["batch"] - c["fd"]) / abs(c["fd"]) if c["fd"] else None
+        c["batch_vs_single_rel"] = abs(c["batch"] - c["single_frame"]) / abs(c["single_frame"]) \
+            if c["single_frame"] else None
+    res["fd_check_theta"] = [checks[k] for k in fd_frames]
     dr.close()
+    db.close()
     return res
 
 

@@ -36,14 +36,15 @@ using namespace bhx::adj;
 
 namespace {
 
-// Checkpoint pool: chunks of kCkChunk records of kCk doubles; link[c] is the
-// chunk its ray filled before c (-1 for the ray's first).  Per ray, the last
-// chunk and the number of records.
-constexpr int kCkChunk = 8;
-constexpr int64_t kMaxPoolChunks = (int64_t(128) << 30) / (kCk * kCkChunk * sizeof(double));   // 128 GiB
+// Checkpoint pool: chunks of 2^lg records of kCk doubles (lg from K: a ray
+// of typical length fills about one chunk); link[c] is the chunk its ray
+// filled before c (-1 for the ray's first).  Per ray, the last chunk and the
+// number of records.
+constexpr int64_t kMaxPoolBytes = int64_t(128) << 30;   // 128 GiB
+inline int chunk_lg(int K) { return K >= 64 ? 2 : (K >= 16 ? 3 : 4); }
 
-// The forward's store of one ray: record j goes to slot j % kCkChunk of the
-// ray's current chunk; a new chunk is taken from the counter every kCkChunk
+// The forward's store of one ray: record j goes to slot j mod 2^lg of the
+// ray's current chunk; a new chunk is taken from the counter every 2^lg
 // records.  Past the pool's end the counter still counts (it tells the host
 // the size needed) and the records are dropped: the ray is marked lost
 // (current chunk kLost).  The lane's current chunk and record count live in
@@ -55,11 +56,13 @@ struct PoolWriter {
     int32_t* link;
     uint32_t* counter;
     uint32_t cap;              // chunks in the pool
+    int32_t lg;
     int4* slot;
     __device__ void start(int32_t ray) { *slot = make_int4(-1, 0, ray, 0); }
     __device__ double* put(int j) {
         int32_t cur = slot->x;
-        if (j % kCkChunk == 0) {
+        const int32_t mask = (1 << lg) - 1;
+        if ((j & mask) == 0) {
             const uint32_t c = atomicAdd(counter, 1u);
             if (c < cap && cur != kLost) {
                 link[c] = cur;
@@ -70,7 +73,7 @@ struct PoolWriter {
         }
         slot->x = cur;
         slot->y = j + 1;
-        return cur == kLost ? nullptr : pool + (static_cast<int64_t>(cur) * kCkChunk + j % kCkChunk) * kCk;
+        return cur == kLost ? nullptr : pool + ((static_cast<int64_t>(cur) << lg) + (j & mask)) * kCk;
     }
 };
 
@@ -79,11 +82,11 @@ struct PoolWriter {
 struct PoolReader {
     const double* pool;
     const int32_t* link;
-    int32_t last, nrec;
+    int32_t last, nrec, lg;
     __device__ const double* get(int j) const {
         int32_t c = last;
-        for (int k = (nrec - 1) / kCkChunk; k > j / kCkChunk; --k) c = link[c];
-        return pool + (static_cast<int64_t>(c) * kCkChunk + j % kCkChunk) * kCk;
+        for (int k = (nrec - 1) >> lg; k > (j >> lg); --k) c = link[c];
+        return pool + ((static_cast<int64_t>(c) << lg) + (j & ((1 << lg) - 1))) * kCk;
     }
 };
 
@@ -92,10 +95,11 @@ struct AdjArgs {
     // forward
     uint32_t* counter;
     const int32_t* qmap;      // queue slot -> ray index
-    double* pool;             // checkpoint chunks: pool_cap x kCkChunk x kCk doubles
+    double* pool;             // checkpoint chunks: pool_cap x 2^ck_lg x kCk doubles
     int32_t* link;            // pool_cap chunk links
     uint32_t* pool_counter;   // chunks taken (may exceed pool_cap: then the pool is too small)
     uint32_t pool_cap;
+    int32_t ck_lg;
     int32_t* ck_last;         // per ray: its last chunk (-1: records lost)
     int32_t* ck_nrec;         // per ray: records written
     uint8_t* cls;
@@ -109,15 +113,23 @@ struct AdjArgs {
     const int32_t* order;     // rays by key, descending: the lit rays first
     double* seg;              // (K + 1) * kSeg doubles per lane, strided by seg_stride
     int64_t seg_stride;
-    double* partial;          // 4 per block
+    double* partial;          // 4 per block (one frame)
     unsigned int* mismatch;
+    // frames of a batch (kernels instantiated for batches): ray r belongs to
+    // frame r / n_per, whose camera is cams[r / n_per]; the backward leaves
+    // each processed ray's d(w I)/d(params) in contrib (4 x n_rays SoA) for
+    // the per-frame sums
+    const FrameCam* cams;
+    int64_t n_per;
+    double* contrib;
 };
 
 struct AdjConst {
     Frame F;
-    RaygenArgs G;
+    FrameCam cam;             // the camera of a single frame
 };
 
+template <bool kBatch>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 k_adj_forward(const AdjConst* __restrict__ C, const AdjArgs P) {
     // A step's conical-jet sample (32 nodes, ~10 steps of work) is not taken in
@@ -129,7 +141,6 @@ k_adj_forward(const AdjConst* __restrict__ C, const AdjArgs P) {
     __shared__ double pend[256 * 17];
     __shared__ double nodes[4][64];
     const Frame& F = C->F;
-    const RaygenArgs& G = C->G;
     const int lane = __lane_id();
     double* my_pend = pend + threadIdx.x * 17;
     double* wave_nodes = nodes[threadIdx.x / 64];
@@ -141,7 +152,7 @@ k_adj_forward(const AdjConst* __restrict__ C, const AdjArgs P) {
     // intensity of a ray that ended, until its last jet sample is in
     __shared__ int4 lane_ray[256];
     __shared__ double lane_idisc[256];
-    PoolWriter wr{P.pool, P.link, P.pool_counter, P.pool_cap, &lane_ray[threadIdx.x]};
+    PoolWriter wr{P.pool, P.link, P.pool_counter, P.pool_cap, P.ck_lg, &lane_ray[threadIdx.x]};
     for (;;) {
         unsigned long long owners = __ballot(s.jet_wait != 0);
         while (owners != 0ull) {
@@ -185,7 +196,12 @@ k_adj_forward(const AdjConst* __restrict__ C, const AdjArgs P) {
             } else {
                 const int32_t ray = P.qmap[r];
                 wr.start(ray);
-                fwd_init(F, G, ray, s, wr);
+                if (kBatch) {
+                    const int64_t frame = ray / P.n_per;
+                    fwd_init(F, P.cams[frame], ray - frame * P.n_per, s, wr);
+                } else {
+                    fwd_init(F, C->cam, ray, s, wr);
+                }
                 live = true;
             }
         }
@@ -200,8 +216,10 @@ k_adj_forward(const AdjConst* __restrict__ C, const AdjArgs P) {
             P.ck_last[ray] = ck.x == kLost ? -1 : ck.x;
             P.ck_nrec[ray] = ck.y;
             P.cls[ray] = (uint8_t)term;
-            P.y[ray] = yo[0]; P.y[N + ray] = yo[1]; P.y[2 * N + ray] = yo[2]; P.y[3 * N + ray] = yo[3];
-            P.y[4 * N + ray] = s.pt; P.y[5 * N + ray] = yo[4]; P.y[6 * N + ray] = yo[5]; P.y[7 * N + ray] = s.pph;
+            if (P.y) {
+                P.y[ray] = yo[0]; P.y[N + ray] = yo[1]; P.y[2 * N + ray] = yo[2]; P.y[3 * N + ray] = yo[3];
+                P.y[4 * N + ray] = s.pt; P.y[5 * N + ray] = yo[4]; P.y[6 * N + ray] = yo[5]; P.y[7 * N + ray] = s.pph;
+            }
             P.steps[ray] = s.nsteps;
             lane_idisc[threadIdx.x] = (term == BHX_DISC) ? disc_intensity(F, yo[1], s.pt, s.pph) : 0.0;
             out_wait = true;
@@ -215,11 +233,11 @@ __global__ void k_iota(int32_t* p, int64_t n) {
     if (i < n) p[i] = static_cast<int32_t>(i);
 }
 
+template <bool kBatch>
 __global__ void __launch_bounds__(256) k_adj_backward(const AdjConst* __restrict__ C, const AdjArgs P) {
     __shared__ double part[4][4];
     __shared__ double jet_lds[4][kJetLds];   // per wave: the jet nodes shared across its lanes
     const Frame& F = C->F;
-    const RaygenArgs& G = C->G;
     const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nl = (int64_t)gridDim.x * blockDim.x;
     double g[4] = {0.0, 0.0, 0.0, 0.0};
@@ -229,9 +247,18 @@ __global__ void __launch_bounds__(256) k_adj_backward(const AdjConst* __restrict
         const int64_t ray = P.order[i];
         const double w = P.w[ray];
         if (w == 0.0) continue;
-        const PoolReader rd{P.pool, P.link, P.ck_last[ray], P.ck_nrec[ray]};
-        mism += bwd_ray(F, G, ray, P.steps[ray], P.cls[ray], w, rd, P.seg + lane, P.seg_stride, g,
-                        F.has_jet ? jet_lds[threadIdx.x / 64] : nullptr);
+        const PoolReader rd{P.pool, P.link, P.ck_last[ray], P.ck_nrec[ray], P.ck_lg};
+        if (kBatch) {
+            // this ray's own contribution, summed per frame afterwards
+            const int64_t frame = ray / P.n_per;
+            double gr[4] = {0.0, 0.0, 0.0, 0.0};
+            mism += bwd_ray(F, P.cams[frame], ray - frame * P.n_per, P.steps[ray], P.cls[ray], w, rd, P.seg + lane,
+                            P.seg_stride, gr, F.has_jet ? jet_lds[threadIdx.x / 64] : nullptr);
+            for (int c = 0; c < 4; ++c) P.contrib[c * P.n_rays + ray] = gr[c];
+        } else {
+            mism += bwd_ray(F, C->cam, ray, P.steps[ray], P.cls[ray], w, rd, P.seg + lane, P.seg_stride, g,
+                            F.has_jet ? jet_lds[threadIdx.x / 64] : nullptr);
+        }
     }
     // block sums in a fixed order
 #pragma unroll
@@ -249,45 +276,83 @@ __global__ void __launch_bounds__(256) k_adj_backward(const AdjConst* __restrict
     }
 }
 
+// Per frame of a batch, the sum of its processed rays' contributions (lit
+// rays with dL/dI != 0), in a fixed order: one block per frame, each thread a
+// fixed stride of the frame's rays, then a fixed tree; deterministic.
+__global__ void __launch_bounds__(256) k_adj_frame_sums(const AdjArgs P, const double* __restrict__ w,
+                                                        double* __restrict__ out) {
+    __shared__ double red[4][256];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * P.n_per;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t r = threadIdx.x; r < P.n_per; r += blockDim.x) {
+        const int64_t ray = base + r;
+        if (P.key[ray] > 0 && w[ray] != 0.0)
+            for (int c = 0; c < 4; ++c) acc[c] += P.contrib[c * P.n_rays + ray];
+    }
+    for (int c = 0; c < 4; ++c) red[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int half = 128; half > 0; half >>= 1) {
+        if ((int)threadIdx.x < half)
+            for (int c = 0; c < 4; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + half];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) out[blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
+}
+
 }  // namespace
 
 struct bhx_adjoint {
     bhx_context* ctx = nullptr;
-    bhx_scene scene{};
+    std::vector<bhx_scene> scenes;   // the frame (one), or the frames of a batch
+    bool is_batch = false;
     int K = 0;
-    int64_t n_rays = 0;
+    int ck_lg = 3;                   // checkpoint records per pool chunk: 2^ck_lg
+    int64_t n_per = 0;               // samples per frame
+    int64_t n_rays = 0;              // samples of all frames
     AdjConst host{};
-    DevBuf cst, pool, link, ck_last, ck_nrec, cls, y, steps, I, key, key_sorted, iota, order, sort_tmp, seg,
-        partial, counters, qmap;
-    uint32_t pool_cap = 0;     // chunks in the checkpoint pool
+    DevBuf cst, cams, pool, link, ck_last, ck_nrec, cls, y, steps, I, key, key_sorted, iota, order, sort_tmp, seg,
+        partial, counters, qmap, contrib, frame_grad;
+    uint32_t pool_cap = 0;           // chunks in the checkpoint pool
     int grid_f = 0, grid_b = 0;
     double fwd_ms = 0.0, bwd_ms = 0.0;
     bool fwd_pending = false;
     unsigned int mismatches = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    size_t chunk_bytes() const { return sizeof(double) * kCk * (size_t(1) << ck_lg); }
 };
 
 namespace {
 
-int adj_set_scene(bhx_adjoint* A, const bhx_scene* s) {
+// Frame constants of the adjoint's frames (with its K): one Frame shared by
+// all of them, a camera per frame.
+int adj_set_scenes(bhx_adjoint* A, const bhx_scene* s, int n) {
     bhx_context* ctx = A->ctx;
-    if (!s) return bhx_fail(ctx, BHX_E_INVALID, "scene is NULL");
-    const int S = std::max(1, s->sqrt_spp);
-    const int64_t n = static_cast<int64_t>(s->width) * s->height * S * S;
-    if (n != A->n_rays) return bhx_fail(ctx, BHX_E_INVALID, "scene size differs from the adjoint's");
-    if (s->max_steps < 1) return bhx_fail(ctx, BHX_E_INVALID, "max_steps must be > 0");
+    if (!s || n < 1) return bhx_fail(ctx, BHX_E_INVALID, "scene is NULL");
+    std::vector<FrameCam> cams(static_cast<size_t>(n));
     AdjConst c;
-    if (!make_frame(*s, A->K, c.F, c.G))
-        return bhx_fail(ctx, BHX_E_INVALID,
-                        "only disc scenes (optionally with the conical jet) without neutron star, Sun or hot spot are "
-                        "differentiable");
-    if (!(s->rtol > 0.0) || !(s->atol > 0.0)) return bhx_fail(ctx, BHX_E_INVALID, "rtol/atol must be > 0");
-    if (!(s->eddington > 0.0)) return bhx_fail(ctx, BHX_E_INVALID, "eddington must be > 0");
+    for (int f = 0; f < n; ++f) {
+        const int S = std::max(1, s[f].sqrt_spp);
+        const int64_t m = static_cast<int64_t>(s[f].width) * s[f].height * S * S;
+        if (m != A->n_per) return bhx_fail(ctx, BHX_E_INVALID, "scene size differs from the adjoint's");
+        if (s[f].max_steps < 1) return bhx_fail(ctx, BHX_E_INVALID, "max_steps must be > 0");
+        Frame F;
+        if (!make_frame(s[f], A->K, F, cams[f]))
+            return bhx_fail(ctx, BHX_E_INVALID,
+                            "only disc scenes (optionally with the conical jet) without neutron star, Sun or hot spot "
+                            "are differentiable");
+        if (!(s[f].rtol > 0.0) || !(s[f].atol > 0.0)) return bhx_fail(ctx, BHX_E_INVALID, "rtol/atol must be > 0");
+        if (!(s[f].eddington > 0.0)) return bhx_fail(ctx, BHX_E_INVALID, "eddington must be > 0");
+        if (f == 0) c.F = F;
+        else if (std::memcmp(&F, &c.F, sizeof F) != 0)
+            return bhx_fail(ctx, BHX_E_INVALID, "the frames of a batch must differ in their camera only");
+    }
+    c.cam = cams[0];
     // the constants are read by kernels of earlier calls: replace them in order
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(A->cst.p, &c, sizeof c, hipMemcpyHostToDevice));
+    if (A->is_batch) HIPCHK(hipMemcpy(A->cams.p, cams.data(), sizeof(FrameCam) * n, hipMemcpyHostToDevice));
     A->host = c;
-    A->scene = *s;
+    A->scenes.assign(s, s + n);
     return BHX_OK;
 }
 
@@ -303,19 +368,26 @@ int adj_record(bhx_adjoint* A, double* I, hipStream_t st) {
     P.link = static_cast<int32_t*>(A->link.p);
     P.pool_counter = reinterpret_cast<uint32_t*>(static_cast<char*>(A->counters.p) + 16);
     P.pool_cap = A->pool_cap;
+    P.ck_lg = A->ck_lg;
     P.ck_last = static_cast<int32_t*>(A->ck_last.p);
     P.ck_nrec = static_cast<int32_t*>(A->ck_nrec.p);
     P.cls = static_cast<uint8_t*>(A->cls.p);
-    P.y = static_cast<double*>(A->y.p);
+    P.y = static_cast<double*>(A->y.p);   // NULL for a batch: hit states are not kept
     P.steps = static_cast<int32_t*>(A->steps.p);
     P.I = I ? I : static_cast<double*>(A->I.p);
     P.key = static_cast<int32_t*>(A->key.p);
     P.n_lit = reinterpret_cast<unsigned int*>(static_cast<char*>(A->counters.p) + 8);
+    P.cams = static_cast<const FrameCam*>(A->cams.p);
+    P.n_per = A->n_per;
     // counters: [0] queue, [8] lit rays, [16] pool chunks taken, [32] replay mismatches
     HIPCHK(launch_zero_u64(static_cast<unsigned long long*>(A->counters.p), 8, st));
     HIPCHK(hipEventRecord(A->ev[0], st));
-    hipLaunchKernelGGL(k_adj_forward, dim3(A->grid_f), dim3(256), 0, st,
-                       static_cast<const AdjConst*>(A->cst.p), P);
+    if (A->is_batch)
+        hipLaunchKernelGGL(k_adj_forward<true>, dim3(A->grid_f), dim3(256), 0, st,
+                           static_cast<const AdjConst*>(A->cst.p), P);
+    else
+        hipLaunchKernelGGL(k_adj_forward<false>, dim3(A->grid_f), dim3(256), 0, st,
+                           static_cast<const AdjConst*>(A->cst.p), P);
     HIPCHK(hipGetLastError());
     // the backward's ray order: lit rays first, most accepted steps first
     // (a stable radix sort on the 15-bit step count: deterministic)
@@ -329,17 +401,38 @@ int adj_record(bhx_adjoint* A, double* I, hipStream_t st) {
     return BHX_OK;
 }
 
-// The adjoint sweep over the last forward's lit rays (synchronises st).
+// A forward that ran out of pool chunks counted what it needed: grow the pool
+// to that and record the same forward again (deterministic: the same rays,
+// steps and intensities; the caller's intensity buffer is not written again).
+int adj_grow_if_short(bhx_adjoint* A, hipStream_t st) {
+    bhx_context* ctx = A->ctx;
+    uint32_t taken = 0;
+    HIPCHK(hipMemcpyAsync(&taken, static_cast<char*>(A->counters.p) + 16, sizeof taken, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (taken <= A->pool_cap) return BHX_OK;
+    if (static_cast<double>(taken) * A->chunk_bytes() > static_cast<double>(kMaxPoolBytes))
+        return bhx_fail(ctx, BHX_E_TOO_LARGE, "checkpoint pool above 128 GiB: raise K");
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(A->pool.ensure(A->chunk_bytes() * static_cast<size_t>(taken)));
+    HIPCHK(A->link.ensure(sizeof(int32_t) * static_cast<size_t>(taken)));
+    A->pool_cap = taken;
+    return adj_record(A, nullptr, st);
+}
+
+// The adjoint sweep over the last forward's lit rays (synchronises st):
+// grad gets 4 values per frame.
 int adj_sweep(bhx_adjoint* A, const double* w, double* grad, hipStream_t st) {
     bhx_context* ctx = A->ctx;
     AdjArgs P{};
     P.n_rays = A->n_rays;
     P.pool = static_cast<double*>(A->pool.p);
     P.link = static_cast<int32_t*>(A->link.p);
+    P.ck_lg = A->ck_lg;
     P.ck_last = static_cast<int32_t*>(A->ck_last.p);
     P.ck_nrec = static_cast<int32_t*>(A->ck_nrec.p);
     P.cls = static_cast<uint8_t*>(A->cls.p);
     P.steps = static_cast<int32_t*>(A->steps.p);
+    P.key = static_cast<int32_t*>(A->key.p);
     P.w = w;
     P.order = static_cast<const int32_t*>(A->order.p);
     P.seg = static_cast<double*>(A->seg.p);
@@ -347,20 +440,38 @@ int adj_sweep(bhx_adjoint* A, const double* w, double* grad, hipStream_t st) {
     P.partial = static_cast<double*>(A->partial.p);
     P.mismatch = reinterpret_cast<unsigned int*>(static_cast<char*>(A->counters.p) + 32);
     P.n_lit = reinterpret_cast<unsigned int*>(static_cast<char*>(A->counters.p) + 8);
+    P.cams = static_cast<const FrameCam*>(A->cams.p);
+    P.n_per = A->n_per;
+    P.contrib = static_cast<double*>(A->contrib.p);
+    const int n_frames = static_cast<int>(A->scenes.size());
     HIPCHK(launch_zero_u32(P.mismatch, 1, st));
     HIPCHK(hipEventRecord(A->ev[2], st));
-    hipLaunchKernelGGL(k_adj_backward, dim3(A->grid_b), dim3(256), 0, st,
-                       static_cast<const AdjConst*>(A->cst.p), P);
+    if (A->is_batch) {
+        hipLaunchKernelGGL(k_adj_backward<true>, dim3(A->grid_b), dim3(256), 0, st,
+                           static_cast<const AdjConst*>(A->cst.p), P);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_adj_frame_sums, dim3(n_frames), dim3(256), 0, st, P, w,
+                           static_cast<double*>(A->frame_grad.p));
+    } else {
+        hipLaunchKernelGGL(k_adj_backward<false>, dim3(A->grid_b), dim3(256), 0, st,
+                           static_cast<const AdjConst*>(A->cst.p), P);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(A->ev[3], st));
-    std::vector<double> part(4 * static_cast<size_t>(A->grid_b));
     unsigned int mism = 0;
-    HIPCHK(hipMemcpyAsync(part.data(), A->partial.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&mism, P.mismatch, sizeof mism, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    for (int c = 0; c < 4; ++c) grad[c] = 0.0;
-    for (int b = 0; b < A->grid_b; ++b)
-        for (int c = 0; c < 4; ++c) grad[c] += part[4 * b + c];
+    if (A->is_batch) {
+        HIPCHK(hipMemcpyAsync(grad, A->frame_grad.p, sizeof(double) * 4 * n_frames, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(&mism, P.mismatch, sizeof mism, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    } else {
+        std::vector<double> part(4 * static_cast<size_t>(A->grid_b));
+        HIPCHK(hipMemcpyAsync(part.data(), A->partial.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(&mism, P.mismatch, sizeof mism, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (int c = 0; c < 4; ++c) grad[c] = 0.0;
+        for (int b = 0; b < A->grid_b; ++b)
+            for (int c = 0; c < 4; ++c) grad[c] += part[4 * b + c];
+    }
     A->mismatches = mism;
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, A->ev[2], A->ev[3]) == hipSuccess) A->bwd_ms = ms;
@@ -369,101 +480,141 @@ int adj_sweep(bhx_adjoint* A, const double* w, double* grad, hipStream_t st) {
     return BHX_OK;
 }
 
-}  // namespace
+// The checkpoint interval: the segment buffer holds K + 1 steps per lane and
+// the pool's chunks 2^chunk_lg(K) records; the constants carry K.
+int adj_set_K(bhx_adjoint* A, int K) {
+    bhx_context* ctx = A->ctx;
+    const int64_t lanes = static_cast<int64_t>(A->grid_b) * 256;
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(A->seg.ensure(sizeof(double) * kSeg * (K + 1) * lanes));
+    A->K = K;
+    A->ck_lg = chunk_lg(K);
+    A->pool_cap = static_cast<uint32_t>(std::min<size_t>(A->pool.bytes / A->chunk_bytes(), A->link.bytes / 4));
+    return adj_set_scenes(A, A->scenes.data(), static_cast<int>(A->scenes.size()));
+}
 
-extern "C" {
-
-int bhx_adjoint_create(bhx_context* ctx, const bhx_scene* s, int32_t K, bhx_adjoint** out) {
+int adj_create(bhx_context* ctx, const bhx_scene* s, int32_t n, int32_t K, bool batch, bhx_adjoint** out) {
     if (!ctx || !out) return BHX_E_INVALID;
     *out = nullptr;
-    if (!s) return bhx_fail(ctx, BHX_E_INVALID, "scene is NULL");
+    if (!s || n < 1) return bhx_fail(ctx, BHX_E_INVALID, "scene is NULL");
+    if (n > 65535) return bhx_fail(ctx, BHX_E_TOO_LARGE, "more than 65535 frames in a batch");
     if (K < 1 || K > 4096) return bhx_fail(ctx, BHX_E_INVALID, "checkpoint interval K must be in [1, 4096]");
-    if (s->width <= 0 || s->height <= 0 || s->max_steps < 1)
+    if (s[0].width <= 0 || s[0].height <= 0 || s[0].max_steps < 1)
         return bhx_fail(ctx, BHX_E_INVALID, "width/height/max_steps must be > 0");
-    const int S = std::max(1, s->sqrt_spp);
-    const int64_t n = static_cast<int64_t>(s->width) * s->height * S * S;
-    if (n > 2000000000LL) return bhx_fail(ctx, BHX_E_TOO_LARGE, "more than 2e9 samples");
-    // first guess of the checkpoint pool: rays of up to 512 accepted steps
-    // (a frame at rtol 1e-10 averages ~300); a forward that finds it short
-    // tells the first backward after it, which grows the pool to the need
-    const int64_t per_ray = (512 + static_cast<int64_t>(K) * kCkChunk - 1) / (static_cast<int64_t>(K) * kCkChunk) + 1;
-    const int64_t chunks = std::min<int64_t>(n * per_ray, kMaxPoolChunks);
+    const int S = std::max(1, s[0].sqrt_spp);
+    const int64_t n_per = static_cast<int64_t>(s[0].width) * s[0].height * S * S;
+    for (int f = 1; f < n; ++f)
+        if (s[f].width != s[0].width || s[f].height != s[0].height || std::max(1, s[f].sqrt_spp) != S)
+            return bhx_fail(ctx, BHX_E_INVALID, "the frames of a batch must have one size");
+    const int64_t n_rays = n_per * n;
+    if (n_rays > 2000000000LL) return bhx_fail(ctx, BHX_E_TOO_LARGE, "more than 2e9 samples");
     HIPCHK(hipSetDevice(ctx->device));
     bhx_adjoint* A = new bhx_adjoint();
     A->ctx = ctx;
+    A->is_batch = batch;
     A->K = K;
-    A->n_rays = n;
+    A->ck_lg = chunk_lg(K);
+    A->n_per = n_per;
+    A->n_rays = n_rays;
     auto bail = [&](int rc) {
         bhx_adjoint_destroy(A);
         return rc;
     };
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adj_forward, 256, 0) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adj_forward<false>, 256, 0) != hipSuccess || per_cu < 1)
         per_cu = 1;
     A->grid_f = per_cu * ctx->n_cu;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adj_backward, 256, 0) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adj_backward<false>, 256, 0) != hipSuccess ||
+        per_cu < 1)
         per_cu = 1;
     A->grid_b = per_cu * ctx->n_cu;
     const int64_t lanes = static_cast<int64_t>(A->grid_b) * 256;
+    // first guess of the checkpoint pool: single frames rays of up to 512
+    // accepted steps (a frame at rtol 1e-10 averages ~300), batches 256; a
+    // forward that finds it short tells the first backward after it, which
+    // grows the pool to the need
+    const int64_t rec = (batch ? 256 : 512) / K + 1;
+    const int64_t per_ray = (rec + (int64_t(1) << A->ck_lg) - 1) >> A->ck_lg;
+    const int64_t chunks = std::min<int64_t>(n_rays * per_ray, kMaxPoolBytes / static_cast<int64_t>(A->chunk_bytes()));
     A->pool_cap = static_cast<uint32_t>(chunks);
     if (A->cst.ensure(sizeof(AdjConst)) != hipSuccess ||
-        A->pool.ensure(sizeof(double) * kCk * kCkChunk * static_cast<size_t>(chunks)) != hipSuccess ||
+        (batch && A->cams.ensure(sizeof(FrameCam) * n) != hipSuccess) ||
+        A->pool.ensure(A->chunk_bytes() * static_cast<size_t>(chunks)) != hipSuccess ||
         A->link.ensure(sizeof(int32_t) * static_cast<size_t>(chunks)) != hipSuccess ||
-        A->ck_last.ensure(sizeof(int32_t) * n) != hipSuccess || A->ck_nrec.ensure(sizeof(int32_t) * n) != hipSuccess ||
-        A->cls.ensure(n) != hipSuccess || A->y.ensure(sizeof(double) * 8 * n) != hipSuccess ||
-        A->steps.ensure(sizeof(int32_t) * n) != hipSuccess || A->I.ensure(sizeof(double) * n) != hipSuccess ||
-        A->order.ensure(sizeof(int32_t) * n) != hipSuccess || A->key.ensure(sizeof(int32_t) * n) != hipSuccess ||
-        A->key_sorted.ensure(sizeof(int32_t) * n) != hipSuccess || A->iota.ensure(sizeof(int32_t) * n) != hipSuccess ||
+        A->ck_last.ensure(sizeof(int32_t) * n_rays) != hipSuccess ||
+        A->ck_nrec.ensure(sizeof(int32_t) * n_rays) != hipSuccess || A->cls.ensure(n_rays) != hipSuccess ||
+        (!batch && A->y.ensure(sizeof(double) * 8 * n_rays) != hipSuccess) ||
+        A->steps.ensure(sizeof(int32_t) * n_rays) != hipSuccess || A->I.ensure(sizeof(double) * n_rays) != hipSuccess ||
+        A->order.ensure(sizeof(int32_t) * n_rays) != hipSuccess || A->key.ensure(sizeof(int32_t) * n_rays) != hipSuccess ||
+        A->key_sorted.ensure(sizeof(int32_t) * n_rays) != hipSuccess ||
+        A->iota.ensure(sizeof(int32_t) * n_rays) != hipSuccess ||
         A->seg.ensure(sizeof(double) * kSeg * (K + 1) * lanes) != hipSuccess ||
         A->partial.ensure(sizeof(double) * 4 * A->grid_b) != hipSuccess ||
-        A->counters.ensure(64) != hipSuccess)
+        (batch && A->contrib.ensure(sizeof(double) * 4 * n_rays) != hipSuccess) ||
+        (batch && A->frame_grad.ensure(sizeof(double) * 4 * n) != hipSuccess) || A->counters.ensure(64) != hipSuccess)
         return bail(bhx_fail(ctx, BHX_E_NOMEM, "adjoint buffers"));
     for (auto& e : A->ev)
         if (hipEventCreate(&e) != hipSuccess) return bail(bhx_fail(ctx, BHX_E_HIP, "hipEventCreate"));
-    // the forward's queue order: the tracer's tile order as a slot -> ray map
+    // the forward's queue: frame after frame, each in the tracer's tile order
     {
-        std::vector<int32_t> order, start;
-        host::tile_queue(s->width, s->height, S, 0, order, start, host::axis_band(*s), s->cam_roll);
-        std::vector<int32_t> map(static_cast<size_t>(n));
-        const int tiles_x = (s->width + 7) / 8, S2 = S * S;
-        for (size_t t = 0; t < order.size(); ++t) {
-            const int tx = order[t] % tiles_x, ty = order[t] / tiles_x;
-            const int x0 = 8 * tx, y0 = 8 * ty;
-            const int tw = std::min(8, s->width - x0), th = std::min(8, s->height - y0);
-            for (int i = 0; i < tw * th * S2; ++i) {
-                const int pin = i / S2, k = i - pin * S2;
-                const int px = x0 + pin % tw, py = y0 + pin / tw;
-                map[static_cast<size_t>(start[t]) + i] = (py * s->width + px) * S2 + k;
+        std::vector<int32_t> map(static_cast<size_t>(n_rays));
+        const int W = s[0].width, H = s[0].height, tiles_x = (W + 7) / 8, S2 = S * S;
+        for (int f = 0; f < n; ++f) {
+            std::vector<int32_t> order, start;
+            host::tile_queue(W, H, S, 0, order, start, host::axis_band(s[f]), s[f].cam_roll);
+            const int64_t base = n_per * f;
+            for (size_t t = 0; t < order.size(); ++t) {
+                const int tx = order[t] % tiles_x, ty = order[t] / tiles_x;
+                const int x0 = 8 * tx, y0 = 8 * ty;
+                const int tw = std::min(8, W - x0), th = std::min(8, H - y0);
+                for (int i = 0; i < tw * th * S2; ++i) {
+                    const int pin = i / S2, k = i - pin * S2;
+                    const int px = x0 + pin % tw, py = y0 + pin / tw;
+                    map[static_cast<size_t>(base + start[t] + i)] = static_cast<int32_t>(base + (py * W + px) * S2 + k);
+                }
             }
         }
-        if (A->qmap.ensure(sizeof(int32_t) * n) != hipSuccess) return bail(bhx_fail(ctx, BHX_E_NOMEM, "queue map"));
-        if (hipMemcpy(A->qmap.p, map.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice) != hipSuccess)
+        if (A->qmap.ensure(sizeof(int32_t) * n_rays) != hipSuccess) return bail(bhx_fail(ctx, BHX_E_NOMEM, "queue map"));
+        if (hipMemcpy(A->qmap.p, map.data(), sizeof(int32_t) * n_rays, hipMemcpyHostToDevice) != hipSuccess)
             return bail(bhx_fail(ctx, BHX_E_HIP, "queue map upload"));
     }
     // ray ids 0..n-1 (sort values) and the radix sort's scratch
-    hipLaunchKernelGGL(k_iota, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, 0,
-                       static_cast<int32_t*>(A->iota.p), n);
+    hipLaunchKernelGGL(k_iota, dim3(static_cast<unsigned>((n_rays + 255) / 256)), dim3(256), 0, 0,
+                       static_cast<int32_t*>(A->iota.p), n_rays);
     if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
         return bail(bhx_fail(ctx, BHX_E_HIP, "k_iota"));
     size_t tmp = 0;
     if (hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp, static_cast<const int32_t*>(A->key.p),
                                                      static_cast<int32_t*>(A->key_sorted.p),
                                                      static_cast<const int32_t*>(A->iota.p),
-                                                     static_cast<int32_t*>(A->order.p), n, 0, 15) != hipSuccess ||
+                                                     static_cast<int32_t*>(A->order.p), n_rays, 0, 15) != hipSuccess ||
         A->sort_tmp.ensure(std::max<size_t>(tmp, 16)) != hipSuccess)
         return bail(bhx_fail(ctx, BHX_E_NOMEM, "radix sort scratch"));
-    const int rc = adj_set_scene(A, s);
+    const int rc = adj_set_scenes(A, s, n);
     if (rc) return bail(rc);
     *out = A;
     return BHX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bhx_adjoint_create(bhx_context* ctx, const bhx_scene* s, int32_t K, bhx_adjoint** out) {
+    return adj_create(ctx, s, 1, K, false, out);
+}
+
+int bhx_adjoint_create_batch(bhx_context* ctx, const bhx_scene* scenes, int32_t n, int32_t K, bhx_adjoint** out) {
+    return adj_create(ctx, scenes, n, K, true, out);
 }
 
 int bhx_adjoint_destroy(bhx_adjoint* A) {
     if (!A) return BHX_OK;
     (void)hipSetDevice(A->ctx->device);
     (void)hipDeviceSynchronize();
-    for (DevBuf* d : {&A->cst, &A->pool, &A->link, &A->ck_last, &A->ck_nrec, &A->cls, &A->y, &A->steps, &A->I, &A->key, &A->key_sorted, &A->iota,
-                      &A->order, &A->sort_tmp, &A->seg, &A->partial, &A->counters, &A->qmap})
+    for (DevBuf* d : {&A->cst, &A->cams, &A->pool, &A->link, &A->ck_last, &A->ck_nrec, &A->cls, &A->y, &A->steps,
+                      &A->I, &A->key, &A->key_sorted, &A->iota, &A->order, &A->sort_tmp, &A->seg, &A->partial,
+                      &A->counters, &A->qmap, &A->contrib, &A->frame_grad})
         d->release();
     for (auto& e : A->ev)
         if (e) (void)hipEventDestroy(e);
@@ -474,8 +625,9 @@ int bhx_adjoint_destroy(bhx_adjoint* A) {
 int bhx_adjoint_set_scene(bhx_adjoint* A, const bhx_scene* s) {
     if (!A) return BHX_E_INVALID;
     bhx_context* ctx = A->ctx;
+    if (A->is_batch) return bhx_fail(ctx, BHX_E_INVALID, "a batch keeps the frames it was created with");
     HIPCHK(hipSetDevice(ctx->device));
-    return adj_set_scene(A, s);
+    return adj_set_scenes(A, s, 1);
 }
 
 int bhx_adjoint_forward(bhx_adjoint* A, double* I, void* stream) {
@@ -488,25 +640,21 @@ int bhx_adjoint_forward(bhx_adjoint* A, double* I, void* stream) {
 int bhx_adjoint_backward(bhx_adjoint* A, const double* w, double* grad, void* stream) {
     if (!A || !w || !grad) return BHX_E_INVALID;
     bhx_context* ctx = A->ctx;
+    if (A->is_batch) return bhx_fail(ctx, BHX_E_INVALID, "a batch's gradients come from bhx_adjoint_backward_batch");
     HIPCHK(hipSetDevice(ctx->device));
     hipStream_t st = static_cast<hipStream_t>(stream);
-    // a forward that ran out of pool chunks counted what it needed: grow the
-    // pool to that and record the same forward again (deterministic: the same
-    // rays, steps and intensities; the caller's intensity buffer is not
-    // written again)
-    uint32_t taken = 0;
-    HIPCHK(hipMemcpyAsync(&taken, static_cast<char*>(A->counters.p) + 16, sizeof taken, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    if (taken > A->pool_cap) {
-        if (static_cast<int64_t>(taken) > kMaxPoolChunks)
-            return bhx_fail(ctx, BHX_E_TOO_LARGE, "checkpoint pool above 128 GiB: raise K");
-        HIPCHK(hipDeviceSynchronize());
-        HIPCHK(A->pool.ensure(sizeof(double) * kCk * kCkChunk * static_cast<size_t>(taken)));
-        HIPCHK(A->link.ensure(sizeof(int32_t) * static_cast<size_t>(taken)));
-        A->pool_cap = taken;
-        const int rc = adj_record(A, nullptr, st);
-        if (rc) return rc;
-    }
+    const int rc = adj_grow_if_short(A, st);
+    if (rc) return rc;
+    return adj_sweep(A, w, grad, st);
+}
+
+int bhx_adjoint_backward_batch(bhx_adjoint* A, const double* w, double* grad, void* stream) {
+    if (!A || !w || !grad) return BHX_E_INVALID;
+    bhx_context* ctx = A->ctx;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int rc = adj_grow_if_short(A, st);
+    if (rc) return rc;
     return adj_sweep(A, w, grad, st);
 }
 
@@ -525,8 +673,52 @@ int bhx_adjoint_copy_rays(bhx_adjoint* A, uint8_t* cls, double* y, int32_t* step
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
     if (cls) HIPCHK(hipMemcpy(cls, A->cls.p, A->n_rays, hipMemcpyDeviceToHost));
-    if (y) HIPCHK(hipMemcpy(y, A->y.p, sizeof(double) * 8 * A->n_rays, hipMemcpyDeviceToHost));
+    if (y && A->y.p) HIPCHK(hipMemcpy(y, A->y.p, sizeof(double) * 8 * A->n_rays, hipMemcpyDeviceToHost));
     if (steps) HIPCHK(hipMemcpy(steps, A->steps.p, sizeof(int32_t) * A->n_rays, hipMemcpyDeviceToHost));
+    return BHX_OK;
+}
+
+int bhx_adjoint_autotune(bhx_adjoint* A, const double* w, const int32_t* Ks, int32_t n_K, int32_t reps,
+                         int32_t* best_K, double* best_ms, void* stream) {
+    if (!A || !w || !Ks || n_K < 1 || n_K > 16 || reps < 1) return BHX_E_INVALID;
+    bhx_context* ctx = A->ctx;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    for (int i = 0; i < n_K; ++i)
+        if (Ks[i] < 1 || Ks[i] > 4096) return bhx_fail(ctx, BHX_E_INVALID, "checkpoint interval K must be in [1, 4096]");
+    int K_best = A->K;
+    double ms_best = 0.0;
+    std::vector<double> grad(4 * A->scenes.size());
+    for (int i = 0; i < n_K; ++i) {
+        int rc = adj_set_K(A, Ks[i]);
+        if (rc) return rc;
+        // one untimed pass (grows the checkpoint pool if this K needs more)
+        rc = adj_record(A, nullptr, st);
+        if (!rc) rc = adj_grow_if_short(A, st);
+        if (!rc) rc = adj_sweep(A, w, grad.data(), st);
+        if (rc) return rc;
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int r = 0; r < reps && !rc; ++r) {
+            rc = adj_record(A, nullptr, st);
+            if (!rc) rc = adj_sweep(A, w, grad.data(), st);
+        }
+        if (rc) return rc;
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / reps;
+        if (i == 0 || ms < ms_best) {
+            ms_best = ms;
+            K_best = Ks[i];
+        }
+    }
+    const int rc = adj_set_K(A, K_best);
+    if (rc) return rc;
+    if (best_K) *best_K = K_best;
+    if (best_ms) *best_ms = ms_best;
+    return BHX_OK;
+}
+
+int bhx_adjoint_get_interval(const bhx_adjoint* A, int32_t* K) {
+    if (!A || !K) return BHX_E_INVALID;
+    *K = A->K;
     return BHX_OK;
 }
 

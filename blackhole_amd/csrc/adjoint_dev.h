@@ -40,14 +40,12 @@ constexpr int kCk = 8;   // checkpoint record: y[6], h, controller memory (beta 
 // Per-frame constants of a differentiable render.
 struct Frame {
     // integrator (src/render.cpp:71-86)
-    double a, a2, r_h, r_stop, rtol, atol, r_escape, h0, r_in, r_out;
+    double a, a2, r_h, r_stop, rtol, atol, r_escape, r_in, r_out;
     int32_t max_steps, K, sg, pad_k;
     // Novikov-Thorne disc (src/disc.h:46-121) and the spin derivatives of its
     // spin-only constants
     double disc_a, flux_scale, dflux_da, eddington;
     double pt_x0, pt_c[3], pt_x[3], d_x0, d_c[3], d_x[3];
-    // d(e00 e03 e11 e22 e33 gl_tt gl_tp gl_rr gl_hh gl_pp)/d(a, r_cam, theta_cam)
-    double dcam[10][3];
     // conical jet (bhx_scene.jet_model = 1), integrated along the ray: its
     // intensity kappa * J joins the disc's (kappa = nu0^alpha x band integral,
     // J the specific intensity at nu0); d ln j_nu0 / d spin from the
@@ -55,6 +53,17 @@ struct Frame {
     int32_t has_jet, pad_;
     double j0, alpha, kappa, dlnj0_da, r_bound;
     double cone_angle, cone_r0, cone_r1, cone_beta, cone_gamma, cone_cos, cone_itaper;
+};
+
+// The camera of a frame: ray generation (pixel map, jitter, ZAMO tetrad and
+// metric at the camera), the integrator's first step h0 = 0.02 max(1, r_cam)
+// (src/render.cpp:86) and the derivatives
+// d(e00 e03 e11 e22 e33 gl_tt gl_tp gl_rr gl_hh gl_pp)/d(a, r_cam, theta_cam).
+// Frames that differ only in their camera share one Frame.
+struct FrameCam {
+    RaygenArgs G;
+    double h0;
+    double dcam[10][3];
 };
 
 // Integrator state of one ray.
@@ -752,7 +761,8 @@ __host__ __device__ inline void ray_pixel(const RaygenArgs& G, int64_t ray, int&
 // The camera photon of a ray and the integrator's start (src/render.cpp:
 // 76-86); checkpoint 0 is written to ck.
 template <class Ck>
-__host__ __device__ inline void fwd_init(const Frame& F, const RaygenArgs& G, int64_t ray, Ray& s, Ck& ck) {
+__host__ __device__ inline void fwd_init(const Frame& F, const FrameCam& cam, int64_t ray, Ray& s, Ck& ck) {
+    const RaygenArgs& G = cam.G;
     int px, py, k;
     ray_pixel(G, ray, px, py, k);
     double d[3];
@@ -766,7 +776,7 @@ __host__ __device__ inline void fwd_init(const Frame& F, const RaygenArgs& G, in
     kerr_rhs(ray_k(F.a, F.a2, pt, pph), s.y[1], s.y[2], s.y[4], s.y[5], f);
     for (int c = 0; c < 6; ++c) s.k1[c] = f.d[c];
     s.c_prev = f.cs;
-    s.h = F.h0;
+    s.h = cam.h0;
     s.lpw = kBeta * kLog1em4;   // err_prev = 1e-4 (geodesic.h:197)
     s.jet = 0.0;
     s.jet_wait = 0;
@@ -918,9 +928,10 @@ __host__ __device__ inline bool same_ck(const Ray& s, const double* ck) {
 // number of replayed segments whose end state differed from the next
 // checkpoint (0 when the replay reproduces the forward exactly).
 template <class Ck>
-__host__ __device__ inline int bwd_ray(const Frame& F, const RaygenArgs& G, int64_t ray, int n_acc, int cls,
+__host__ __device__ inline int bwd_ray(const Frame& F, const FrameCam& cam, int64_t ray, int n_acc, int cls,
                                        double w, const Ck& ck, double* seg, int64_t stride, double grad[4],
                                        double* lds = nullptr) {
+    const RaygenArgs& G = cam.G;
     int mismatch = 0;
     int px, py, k;
     ray_pixel(G, ray, px, py, k);
@@ -1001,7 +1012,7 @@ __host__ __device__ inline int bwd_ray(const Frame& F, const RaygenArgs& G, int6
     D3 e[10];
     for (int i = 0; i < 10; ++i) {
         e[i] = D3(base[i]);
-        for (int m = 0; m < 3; ++m) e[i].d[m] = F.dcam[i][m];
+        for (int m = 0; m < 3; ++m) e[i].d[m] = cam.dcam[i][m];
     }
     D3 pt, pr, pth, pph;
     camera_photon_t<D3>(e[0], e[1], e[2], e[3], e[4], e[5], e[6], e[7], e[8], e[9], d, pt, pr, pth, pph);

@@ -32,16 +32,18 @@ T circ_energy_t(const T& a, const T& r, int sg) {
 
 // Frame constants of scene s with checkpoints every K accepted steps.
 // Returns false for scenes this render cannot differentiate.
-inline bool make_frame(const bhx_scene& s, int K, Frame& F, RaygenArgs& G) {
+inline bool make_frame(const bhx_scene& s, int K, Frame& F, FrameCam& cam) {
     if (!s.disc_enabled || s.ns_enabled || s.sun_enabled || s.hs_enabled) return false;
     if (s.jet_enabled && s.jet_model != 1) return false;
     if (K < 1 || s.max_steps < 1) return false;
     std::memset(&F, 0, sizeof F);
-    G = host::raygen_args(s);
+    std::memset(&cam, 0, sizeof cam);
+    cam.G = host::raygen_args(s);
     const TraceArgs T = host::trace_args(s);
     const ShadeArgs S = host::shade_args(s);
     F.a = T.a; F.a2 = T.a2; F.r_h = T.r_h; F.r_stop = T.r_stop; F.rtol = T.rtol; F.atol = T.atol;
-    F.r_escape = T.r_escape; F.h0 = T.h0; F.r_in = T.r_in; F.r_out = T.r_out;
+    F.r_escape = T.r_escape; F.r_in = T.r_in; F.r_out = T.r_out;
+    cam.h0 = T.h0;
     F.max_steps = T.max_steps;
     F.K = K;
     F.sg = S.disc_sg;
@@ -118,7 +120,7 @@ inline bool make_frame(const bhx_scene& s, int K, Frame& F, RaygenArgs& G) {
         const D3 e[10] = {1.0 / alpha, omega / alpha, sqrt(Dl / Sg), 1.0 / sqrt(Sg), sqrt(Sg / A) / sn,
                           -(1.0 - 2.0 * r / Sg), -2.0 * a * r * sn2 / Sg, Sg / Dl, Sg, A * sn2 / Sg};
         for (int i = 0; i < 10; ++i)
-            for (int m = 0; m < 3; ++m) F.dcam[i][m] = e[i].d[m];
+            for (int m = 0; m < 3; ++m) cam.dcam[i][m] = e[i].d[m];
     }
     return true;
 }

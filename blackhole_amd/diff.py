@@ -36,11 +36,13 @@ def with_params(scene, p):
     """Copy of scene with (spin, cam_theta, eddington, cam_r) = p; r_escape
     and the rest of the camera stay as they are.  The conical jet's emission,
     j_nu0 ~ phi_flux^2 eps_rad x (a function of spin and accretion rate), is
-    carried to the new spin and rate at the scene's phi_flux^2 eps_rad."""
+    carried to the new spin and rate at the scene's phi_flux^2 eps_rad (and
+    kept as it is, bit for bit, when spin and rate are the scene's)."""
     s = Scene()
     C.pointer(s)[0] = scene
     s.spin, s.cam_theta, s.eddington, s.cam_r = (float(v) for v in p)
-    if s.jet_enabled and s.jet_model == 1 and scene.jet_j_nu0 > 0.0:
+    moved = (s.spin, s.eddington) != (scene.spin, scene.eddington)
+    if moved and s.jet_enabled and s.jet_model == 1 and scene.jet_j_nu0 > 0.0:
         old = Scene()
         C.pointer(old)[0] = scene
         scale = scene.jet_j_nu0 / _unit_cone_jet(old)
@@ -110,6 +112,20 @@ class DiscRender:
             self.tr._stream(None)), "bhx_adjoint_copy_rays")
         return out
 
+    def autotune(self, w, Ks=(16, 32, 64, 128, 256), reps=2):
+        """Pick the fastest checkpoint interval among Ks for this frame (timed
+        gradient passes with dL/dI = w); returns (K, ms per pass).  The next
+        backward needs a new forward."""
+        w = w.contiguous()
+        arr = (C.c_int32 * len(Ks))(*Ks)
+        k, ms = C.c_int32(), C.c_double()
+        native.check(self.tr.ctx, self.lib.bhx_adjoint_autotune(self.adj, C.c_void_p(w.data_ptr()), arr, len(Ks),
+                                                                int(reps), C.byref(k), C.byref(ms),
+                                                                self.tr._stream(None)), "bhx_adjoint_autotune")
+        self.K = k.value
+        self.generation += 1
+        return k.value, ms.value
+
     def info(self):
         ck, disc, mism = C.c_int64(), C.c_int64(), C.c_int64()
         f, b = C.c_double(), C.c_double()
@@ -121,6 +137,75 @@ class DiscRender:
     # ---- autograd ------------------------------------------------------------
     def __call__(self, p):
         return _DiscImage.apply(p, self)
+
+
+class DiscBatch:
+    """One bhx_adjoint over a batch of frames of one size and one physics that
+    differ in their camera only (a camera sweep): one forward traces every
+    frame's rays in one persistent launch, one backward gives each frame its
+    own gradient.
+
+        db = DiscBatch(tracer, frames, K=64)
+        I = db.forward_raw()          # (n_frames, samples) device tensor
+        g = db.backward_raw(w)        # (n_frames, 4): dL_f/d(spin, cam_theta, eddington, cam_r)
+
+    w holds dL/dI for every frame's samples (same layout as I); frame f's row
+    of g is the gradient of the part of L its own samples carry."""
+
+    def __init__(self, tracer, scenes, K=64):
+        self.tr = tracer
+        self.lib = tracer.lib
+        self.scenes = list(scenes)
+        self.n = len(self.scenes)
+        self.n_per = self.scenes[0].n_rays
+        self.K = int(K)
+        self.generation = 0
+        self.adj = C.c_void_p()
+        arr = (Scene * self.n)(*self.scenes)
+        native.check(tracer.ctx, self.lib.bhx_adjoint_create_batch(tracer.ctx, arr, self.n, self.K,
+                                                                   C.byref(self.adj)), "bhx_adjoint_create_batch")
+
+    def close(self):
+        if self.adj:
+            self.lib.bhx_adjoint_destroy(self.adj)
+            self.adj = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forward_raw(self, out=None):
+        """I of every frame's samples, (n_frames, samples) on the device; records checkpoints."""
+        t = self.tr.torch
+        if out is None:
+            out = t.empty((self.n, self.n_per), dtype=t.float64, device=f"cuda:{self.tr.device}")
+        native.check(self.tr.ctx, self.lib.bhx_adjoint_forward(self.adj, C.c_void_p(out.data_ptr()),
+                                                               self.tr._stream(None)), "bhx_adjoint_forward")
+        self.generation += 1
+        return out
+
+    def backward_raw(self, w):
+        """Per-frame dL/d(spin, cam_theta, eddington, cam_r), (n_frames, 4),
+        for w = dL/dI of the last forward's samples."""
+        w = w.contiguous()
+        g = (C.c_double * (4 * self.n))()
+        native.check(self.tr.ctx, self.lib.bhx_adjoint_backward_batch(self.adj, C.c_void_p(w.data_ptr()), g,
+                                                                      self.tr._stream(None)),
+                     "bhx_adjoint_backward_batch")
+        return np.array(g[:]).reshape(self.n, 4)
+
+    def rays(self):
+        """Class and accepted steps of the last forward's rays, (n_frames, samples) host copies."""
+        out = {"cls": np.zeros((self.n, self.n_per), np.uint8), "steps": np.zeros((self.n, self.n_per), np.int32)}
+        native.check(self.tr.ctx, self.lib.bhx_adjoint_copy_rays(
+            self.adj, out["cls"].ctypes.data_as(C.c_void_p), None, out["steps"].ctypes.data_as(C.c_void_p),
+            self.tr._stream(None)), "bhx_adjoint_copy_rays")
+        return out
+
+    autotune = DiscRender.autotune
+    info = DiscRender.info
 
 
 def _image_from_samples(tr, I, scene):

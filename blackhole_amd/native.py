@@ -23,6 +23,7 @@ SYMBOLS = (
     "bhx_adjoint_destroy", "bhx_adjoint_set_scene", "bhx_adjoint_forward", "bhx_adjoint_backward",
     "bhx_adjoint_rays", "bhx_adjoint_copy_rays", "bhx_adjoint_info", "bhx_png_encode", "bhx_png_encode_dev",
     "bhx_apng_encode", "bhx_apng_encode_dev", "bhx_bytes_free", "bhx_trace_noalloc_dev", "bhx_uniform_sky",
+    "bhx_adjoint_autotune", "bhx_adjoint_get_interval", "bhx_adjoint_create_batch", "bhx_adjoint_backward_batch",
 )
 
 
@@ -73,11 +74,16 @@ def lib():
     L.bhx_batch_autotune.argtypes = [P, C.c_int32, C.POINTER(Launch), C.POINTER(C.c_double), P]
     L.bhx_adjoint_create.argtypes = [P, C.POINTER(Scene), C.c_int32, C.POINTER(P)]
     L.bhx_adjoint_destroy.argtypes = [P]
+    L.bhx_adjoint_create_batch.argtypes = [P, C.POINTER(Scene), C.c_int32, C.c_int32, C.POINTER(P)]
+    L.bhx_adjoint_backward_batch.argtypes = [P, P, P, P]
     L.bhx_adjoint_set_scene.argtypes = [P, C.POINTER(Scene)]
     L.bhx_adjoint_forward.argtypes = [P, P, P]
     L.bhx_adjoint_backward.argtypes = [P, P, P, P]
     L.bhx_adjoint_rays.argtypes = [P, C.POINTER(Rays)]
     L.bhx_adjoint_copy_rays.argtypes = [P, P, P, P, P]
+    L.bhx_adjoint_autotune.argtypes = [P, P, C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_double), P]
+    L.bhx_adjoint_get_interval.argtypes = [P, C.POINTER(C.c_int32)]
     L.bhx_adjoint_info.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                    C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.bhx_png_encode.argtypes = [P, P, C.c_int32, C.c_int32, C.POINTER(Bytes)]

@@ -318,8 +318,9 @@ int bhx_batch_autotune(bhx_batch *batch, int32_t reps, bhx_launch *best, double 
  * dL/dI per sample, the gradient of L with respect to (spin, cam_theta [rad],
  * eddington, cam_r).  The forward records a checkpoint of the integrator
  * state every K accepted steps (64 bytes each) into a device pool of
- * 8-record chunks that each ray takes as it goes, so the pool holds about
- * sum over rays of (steps / K + 1) x 64 bytes, whatever max_steps is; a
+ * chunks that each ray takes as it goes (16 records for K < 16, 8 for
+ * K < 64, else 4), so the pool holds about sum over rays of
+ * (steps / K + 1) x 64 bytes, whatever max_steps is; a
  * forward that finds the pool short still completes its image, and the next
  * backward grows the pool to the forward's count and records the same
  * forward again before it sweeps; the backward replays each K-step segment from its checkpoint with
@@ -349,6 +350,33 @@ int bhx_adjoint_backward(bhx_adjoint *adj, const double *w, double *grad, void *
 int bhx_adjoint_rays(const bhx_adjoint *adj, bhx_rays *rays);
 /* The same copied to HOST buffers (any may be NULL; synchronises `stream`). */
 int bhx_adjoint_copy_rays(bhx_adjoint *adj, uint8_t *cls, double *y, int32_t *steps, void *stream);
+/* Autotuner of the checkpoint interval: for each of the n_K candidates Ks
+ * (at most 16), one untimed and `reps` timed gradient passes (recording
+ * forward + backward with dL/dI = w, device) at the adjoint's current scene;
+ * leaves the fastest K set and returns it and its time per pass [ms].  The
+ * gradient does not depend on K (the replay reproduces every checkpoint).
+ * Synchronous; the next backward needs a new forward. */
+int bhx_adjoint_autotune(bhx_adjoint *adj, const double *w, const int32_t *Ks, int32_t n_K, int32_t reps,
+                         int32_t *best_K, double *best_ms, void *stream);
+/* The checkpoint interval in use. */
+int bhx_adjoint_get_interval(const bhx_adjoint *adj, int32_t *K);
+/* A batch of n frames of one size and one physics (spin, disc, jet,
+ * integrator, cam_r) that differ in their camera only (cam_theta, cam_phi,
+ * fov, roll, jitter seed): one persistent forward over all n x samples rays,
+ * so the GPU stays full across frames, and one backward that gives each
+ * frame its own gradient.  I and w hold the frames back to back (frame f's
+ * samples at f x samples_per_frame, each in bhx.h's index order); grad (host,
+ * 4 x n doubles) gets frame f's dL_f/d(spin, cam_theta, eddington, cam_r) at
+ * 4 f, L_f the part of L that frame f's samples carry: sum over f of
+ * frame f's own gradient, not a shared parameter's total.  Each frame's
+ * gradient is its rays' contributions summed in a fixed order
+ * (deterministic); it equals the single-frame backward's to rounding (the
+ * sums run in another order).  Hit states are not kept (bhx_adjoint_rays
+ * gives y = NULL); bhx_adjoint_set_scene and bhx_adjoint_backward refuse a
+ * batch.  At most 65535 frames and 2e9 samples. */
+int bhx_adjoint_create_batch(bhx_context *ctx, const bhx_scene *scenes, int32_t n, int32_t K,
+                             bhx_adjoint **out);
+int bhx_adjoint_backward_batch(bhx_adjoint *adj, const double *w, double *grad, void *stream);
 /* Checkpoint pool size in bytes, lit rays (I > 0) of the last forward, replayed segments
  * that did not reproduce their checkpoint (0 expected), kernel times. */
 int bhx_adjoint_info(bhx_adjoint *adj, int64_t *ck_bytes, int64_t *lit_rays, int64_t *mismatches,

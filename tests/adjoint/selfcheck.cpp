@@ -193,7 +193,7 @@ static bhx_scene with_param(const bhx_scene& sc, int p, double v) {
 // Full adaptive forward of one ray: its intensity (disc plus jet).
 static double ray_I(const bhx_scene& sc, int K, int64_t ray, int* cls, int* n_acc) {
     Frame F;
-    RaygenArgs G;
+    FrameCam G;
     make_frame(sc, K, F, G);
     std::vector<double> ck(static_cast<size_t>(F.max_steps / K + 1) * kCk);
     FlatCk store{ck.data()};
@@ -213,7 +213,7 @@ static double ray_I(const bhx_scene& sc, int K, int64_t ray, int* cls, int* n_ac
 // whose class does not change.
 static void check_ray(const char* name, const bhx_scene& sc, int K, int64_t stride, int n_max) {
     Frame F;
-    RaygenArgs G;
+    FrameCam G;
     if (!make_frame(sc, K, F, G)) {
         std::printf("%s 0 1 mismatch 0\n", name);
         return;

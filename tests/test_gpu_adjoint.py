@@ -203,3 +203,120 @@ def test_full_frame_at_K16(tracer):
     assert info["mismatches"] == 0 and np.all(np.isfinite(g))
     assert info["ck_bytes"] < 8e9
     dr.close()
+
+
+def test_checkpoint_interval_autotuner(tracer):
+    """The tuner leaves one of its candidates set; the gradient at the tuned
+    K equals the gradient at K = 64 bit for bit."""
+    import torch
+    sc = _scene(48)
+    w = torch.linspace(-1.0, 1.0, sc.n_rays, dtype=torch.float64, device=f"cuda:{tracer.device}")
+    ref = DiscRender(tracer, sc, K=64)
+    ref.forward_raw(P0)
+    g64 = ref.backward_raw(w)
+    ref.close()
+    dr = DiscRender(tracer, sc, K=7)
+    dr.forward_raw(P0)
+    K, ms = dr.autotune(w, Ks=(16, 32, 64, 128, 256), reps=1)
+    print(f"[autotune K] {K} {ms:.3f} ms", flush=True)
+    assert K in (16, 32, 64, 128, 256) and ms > 0.0
+    dr.forward_raw(P0)
+    assert np.array_equal(dr.backward_raw(w), g64)
+    assert dr.info()["mismatches"] == 0
+    dr.close()
+
+
+def _sweep_frames(n, width, picks, rtol=None):
+    frames = S.quasar_sweep(n, width, width, stars=False)
+    out = []
+    for i in picks:
+        sc = frames[i]
+        if rtol is not None:
+            sc.rtol, sc.atol = rtol, rtol * 1e-2
+        out.append(sc)
+    return out
+
+
+def test_batch_matches_single_frames(tracer):
+    """Three sweep frames (0.5, 45 and 90 deg; disc + conical jet) through one
+    batched adjoint: each frame's intensities, classes and step counts equal
+    its single-frame adjoint's bit for bit (the same rays through the same
+    arithmetic), and each frame's gradient equals the single-frame gradient
+    with the same dL/dI to rounding (the same per-ray terms, summed in
+    another order)."""
+    import torch
+    from blackhole_amd.diff import DiscBatch
+    frames = _sweep_frames(9, 64, (0, 4, 8))
+    n_per = frames[0].n_rays
+    db = DiscBatch(tracer, frames, K=16)
+    I = db.forward_raw()
+    r = db.rays()
+    w = torch.linspace(-1.0, 1.0, 3 * n_per, dtype=torch.float64, device=I.device).view(3, n_per)
+    gb = db.backward_raw(w)
+    info = db.info()
+    print(f"[batch] {gb.tolist()} {info}", flush=True)
+    assert info["mismatches"] == 0 and info["lit_rays"] > 0
+    for f, sc in enumerate(frames):
+        dr = DiscRender(tracer, sc, K=16)
+        If = dr.forward_raw((sc.spin, sc.cam_theta, sc.eddington, sc.cam_r))
+        rf = dr.rays()
+        assert torch.equal(I[f], If), f
+        assert np.array_equal(r["cls"][f], rf["cls"]) and np.array_equal(r["steps"][f], rf["steps"]), f
+        gs = dr.backward_raw(w[f])
+        rel = np.abs(gb[f] - gs) / np.maximum(np.abs(gs), 1e-300)
+        print(f"[batch frame {f}] single {gs} rel {rel}", flush=True)
+        assert np.all(rel < 1e-11), (f, rel)
+        assert dr.info()["mismatches"] == 0
+        dr.close()
+    # a frame whose dL/dI is zero gets a zero gradient; the others keep theirs
+    w2 = w.clone()
+    w2[1] = 0.0
+    g2 = db.backward_raw(w2)
+    assert np.all(g2[1] == 0.0) and np.array_equal(g2[0], gb[0]) and np.array_equal(g2[2], gb[2])
+    db.close()
+
+
+def test_batch_pool_grows(tracer):
+    """At rtol 1e-12 the batch's first pool guess is short: the backward grows
+    it and records the same forward again; the gradients equal those of a
+    second forward and backward with the grown pool, bit for bit."""
+    import torch
+    from blackhole_amd.diff import DiscBatch
+    frames = _sweep_frames(5, 32, (1, 3), rtol=1e-12)
+    db = DiscBatch(tracer, frames, K=16)
+    I = db.forward_raw()
+    before = db.info()["ck_bytes"]
+    w = torch.linspace(-1.0, 1.0, I.numel(), dtype=torch.float64, device=I.device).view_as(I)
+    g1 = db.backward_raw(w)
+    info = db.info()
+    print(f"[batch pool] {before} -> {info['ck_bytes']} bytes, mean steps {db.rays()['steps'].mean():.0f}",
+          flush=True)
+    assert info["ck_bytes"] > before and info["mismatches"] == 0
+    I2 = db.forward_raw()
+    g2 = db.backward_raw(w)
+    assert torch.equal(I, I2) and np.array_equal(g1, g2)
+    db.close()
+
+
+def test_batch_refusals(tracer):
+    """Frames that differ in more than their camera, or in size, are refused;
+    a batch refuses a new scene and the single-frame backward."""
+    import ctypes as C
+    import torch
+    from blackhole_amd import BhxError
+    from blackhole_amd.diff import DiscBatch
+    a, b = _sweep_frames(3, 32, (0, 2))
+    b2 = with_params(b, (0.9, b.cam_theta, b.eddington, b.cam_r))
+    with pytest.raises(BhxError, match="camera only"):
+        DiscBatch(tracer, [a, b2])
+    c = _sweep_frames(3, 40, (2,))[0]
+    with pytest.raises(BhxError, match="one size"):
+        DiscBatch(tracer, [a, c])
+    db = DiscBatch(tracer, [a, b], K=16)
+    I = db.forward_raw()
+    assert tracer.lib.bhx_adjoint_set_scene(db.adj, C.byref(a)) != 0
+    g = (C.c_double * 4)()
+    w = torch.zeros_like(I)
+    assert tracer.lib.bhx_adjoint_backward(db.adj, C.c_void_p(w.data_ptr()), g, None) != 0
+    assert np.all(db.backward_raw(w) == 0.0)
+    db.close()
