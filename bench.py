@@ -384,14 +384,18 @@ def sweep_leg(tr, torch, n_frames=120, width=1024, K=64, rounds=2, grad=True, fd
     I = db.forward_raw()
     torch.cuda.synchronize()
     t_fwd = time.perf_counter()
+    # dL_k/dI_k, 8 frames at a time: L = sum_k L_k with each frame's target
+    # (the blurred previous frame) held fixed
     w = torch.empty_like(I)
     prev = torch.zeros(width, width, dtype=torch.float64, device=I.device)
-    for k in range(n_frames):
-        Ik = I[k].view(width, width).clone().requires_grad_(True)
-        L = ((gaussian_blur(Ik) - gaussian_blur(prev)) ** 2).sum()
+    for g0 in range(0, n_frames, 8):
+        Ig = I[g0:g0 + 8].view(-1, width, width).clone().requires_grad_(True)
+        Bg = gaussian_blur(Ig)
+        Tg = torch.cat([prev[None], Bg[:-1].detach()])
+        L = ((Bg - Tg) ** 2).sum()
         L.backward()
-        w[k] = Ik.grad.reshape(-1)
-        prev = I[k].view(width, width)
+        w[g0:g0 + 8] = Ig.grad.view(-1, I.shape[1])
+        prev = Bg[-1].detach()
     torch.cuda.synchronize()
     t_w = time.perf_counter()
     g = db.backward_raw(w)

@@ -44,26 +44,31 @@ constexpr int64_t kMaxPoolBytes = int64_t(128) << 30;   // 128 GiB
 inline int chunk_lg(int K) { return K >= 64 ? 2 : (K >= 16 ? 3 : 4); }
 
 // The forward's store of one ray: record j goes to slot j mod 2^lg of the
-// ray's current chunk; a new chunk is taken from the counter every 2^lg
-// records.  Past the pool's end the counter still counts (it tells the host
-// the size needed) and the records are dropped: the ray is marked lost
-// (current chunk kLost).  The lane's current chunk and record count live in
-// an LDS slot (x, y; z holds the lane's ray index): they are touched once
-// every K steps, and registers are what the forward is short of.
+// ray's current chunk.  A ray's first chunk is chunk q, q its queue slot
+// (the pool's first n_rays chunks: starting a ray takes no atomic); every
+// further 2^lg records take a chunk n_rays + c, c from the counter.  Past the
+// pool's end the counter still counts (it tells the host the size needed)
+// and the records are dropped: the ray is marked lost (current chunk kLost).
+// The lane's current chunk and record count live in an LDS slot (x, y; z
+// holds the lane's ray index): they are touched once every K steps, and
+// registers are what the forward is short of.
 constexpr int32_t kLost = -2;
 struct PoolWriter {
     double* pool;
     int32_t* link;
     uint32_t* counter;
-    uint32_t cap;              // chunks in the pool
+    uint32_t cap;              // chunks in the pool (>= n_first)
+    uint32_t n_first;          // first chunks: one per queue slot
     int32_t lg;
     int4* slot;
-    __device__ void start(int32_t ray) { *slot = make_int4(-1, 0, ray, 0); }
+    __device__ void start(int32_t ray, int32_t q) { *slot = make_int4(q, 0, ray, 0); }
     __device__ double* put(int j) {
         int32_t cur = slot->x;
         const int32_t mask = (1 << lg) - 1;
-        if ((j & mask) == 0) {
-            const uint32_t c = atomicAdd(counter, 1u);
+        if (j == 0) {
+            link[cur] = -1;
+        } else if ((j & mask) == 0) {
+            const uint64_t c = static_cast<uint64_t>(n_first) + atomicAdd(counter, 1u);
             if (c < cap && cur != kLost) {
                 link[c] = cur;
                 cur = static_cast<int32_t>(c);
@@ -97,8 +102,11 @@ struct AdjArgs {
     const int32_t* qmap;      // queue slot -> ray index
     double* pool;             // checkpoint chunks: pool_cap x 2^ck_lg x kCk doubles
     int32_t* link;            // pool_cap chunk links
-    uint32_t* pool_counter;   // chunks taken (may exceed pool_cap: then the pool is too small)
+    uint32_t* pool_counter;   // chunks taken beyond the first ones (the pool is short when
+                              // n_rays + this exceeds pool_cap)
     uint32_t pool_cap;
+    int32_t q_chunk;          // queue slots a wave takes at once
+    int32_t refill;           // idle lanes a wave waits for before it refills them
     int32_t ck_lg;
     int32_t* ck_last;         // per ray: its last chunk (-1: records lost)
     int32_t* ck_nrec;         // per ray: records written
@@ -147,12 +155,20 @@ k_adj_forward(const AdjConst* __restrict__ C, const AdjArgs P) {
     Ray s;
     s.jet_wait = 0;
     s.pt = s.pph = s.jet = 0.0;
-    bool live = false, done = false, out_wait = false;
+    bool live = false, out_wait = false;
     // per lane in LDS: checkpoint chunk, record count, ray index; the disc
     // intensity of a ray that ended, until its last jet sample is in
     __shared__ int4 lane_ray[256];
     __shared__ double lane_idisc[256];
-    PoolWriter wr{P.pool, P.link, P.pool_counter, P.pool_cap, P.ck_lg, &lane_ray[threadIdx.x]};
+    PoolWriter wr{P.pool, P.link, P.pool_counter, P.pool_cap, static_cast<uint32_t>(P.n_rays), P.ck_lg,
+                  &lane_ray[threadIdx.x]};
+    // the wave's share of the queue: slots [c_next, c_end), taken q_chunk at
+    // a time; idle lanes get consecutive slots (ballot + prefix rank), once
+    // `refill` of them wait or the whole wave is idle, so that the rays'
+    // set-up (camera, first right-hand side, checkpoint 0) runs on many lanes
+    // at once rather than once per finished lane
+    int64_t c_next = 0, c_end = 0;
+    bool q_done = false;
     for (;;) {
         unsigned long long owners = __ballot(s.jet_wait != 0);
         while (owners != 0ull) {
@@ -189,22 +205,42 @@ k_adj_forward(const AdjConst* __restrict__ C, const AdjArgs P) {
             if (I > 0.0) atomicAdd(P.n_lit, 1u);
             out_wait = false;
         }
-        if (!live && !done) {
-            const uint32_t r = atomicAdd(P.counter, 1u);
-            if ((int64_t)r >= P.n_rays) {
-                done = true;
-            } else {
-                const int32_t ray = P.qmap[r];
-                wr.start(ray);
-                if (kBatch) {
-                    const int64_t frame = ray / P.n_per;
-                    fwd_init(F, P.cams[frame], ray - frame * P.n_per, s, wr);
-                } else {
-                    fwd_init(F, C->cam, ray, s, wr);
+        unsigned long long idle = __ballot(!live);
+        const int n_idle = __popcll(idle);
+        if (!q_done && (n_idle >= P.refill || n_idle == 64)) {
+            while (idle != 0ull) {
+                if (c_next >= c_end) {
+                    uint32_t b = 0;
+                    if (lane == 0) b = atomicAdd(P.counter, static_cast<uint32_t>(P.q_chunk));
+                    b = __builtin_amdgcn_readfirstlane(b);
+                    if (static_cast<int64_t>(b) >= P.n_rays) {
+                        q_done = true;
+                        break;
+                    }
+                    c_next = b;
+                    c_end = min(P.n_rays, static_cast<int64_t>(b) + P.q_chunk);
                 }
-                live = true;
+                const int64_t avail = c_end - c_next;
+                const int rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(idle >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(idle), 0u));
+                if (!live && rank < avail) {
+                    const int64_t q = c_next + rank;
+                    const int32_t ray = P.qmap[q];
+                    wr.start(ray, static_cast<int32_t>(q));
+                    if (kBatch) {
+                        const int64_t frame = ray / P.n_per;
+                        fwd_init(F, P.cams[frame], ray - frame * P.n_per, s, wr);
+                    } else {
+                        fwd_init(F, C->cam, ray, s, wr);
+                    }
+                    live = true;
+                }
+                c_next += min(static_cast<int64_t>(__popcll(idle)), avail);
+                idle = __ballot(!live);
             }
         }
+        // a refill runs whenever all 64 lanes are idle and only stops with
+        // every lane live or the queue drained: no live lane, no work left
         if (__ballot(live) == 0ull) break;
         if (!live) continue;
         double yo[6], se = 0.0;
@@ -312,7 +348,9 @@ struct bhx_adjoint {
     AdjConst host{};
     DevBuf cst, cams, pool, link, ck_last, ck_nrec, cls, y, steps, I, key, key_sorted, iota, order, sort_tmp, seg,
         partial, counters, qmap, contrib, frame_grad;
-    uint32_t pool_cap = 0;           // chunks in the checkpoint pool
+    uint32_t pool_cap = 0;           // chunks in the checkpoint pool (>= n_rays: a first chunk per ray)
+    int32_t q_chunk = 32;            // the forward's queue: slots per wave take
+    int32_t refill = 4;              // idle lanes a wave waits for before it refills them
     int grid_f = 0, grid_b = 0;
     double fwd_ms = 0.0, bwd_ms = 0.0;
     bool fwd_pending = false;
@@ -368,6 +406,8 @@ int adj_record(bhx_adjoint* A, double* I, hipStream_t st) {
     P.link = static_cast<int32_t*>(A->link.p);
     P.pool_counter = reinterpret_cast<uint32_t*>(static_cast<char*>(A->counters.p) + 16);
     P.pool_cap = A->pool_cap;
+    P.q_chunk = A->q_chunk;
+    P.refill = A->refill;
     P.ck_lg = A->ck_lg;
     P.ck_last = static_cast<int32_t*>(A->ck_last.p);
     P.ck_nrec = static_cast<int32_t*>(A->ck_nrec.p);
@@ -406,16 +446,17 @@ int adj_record(bhx_adjoint* A, double* I, hipStream_t st) {
 // steps and intensities; the caller's intensity buffer is not written again).
 int adj_grow_if_short(bhx_adjoint* A, hipStream_t st) {
     bhx_context* ctx = A->ctx;
-    uint32_t taken = 0;
-    HIPCHK(hipMemcpyAsync(&taken, static_cast<char*>(A->counters.p) + 16, sizeof taken, hipMemcpyDeviceToHost, st));
+    uint32_t more = 0;
+    HIPCHK(hipMemcpyAsync(&more, static_cast<char*>(A->counters.p) + 16, sizeof more, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    if (taken <= A->pool_cap) return BHX_OK;
-    if (static_cast<double>(taken) * A->chunk_bytes() > static_cast<double>(kMaxPoolBytes))
+    const int64_t need = A->n_rays + static_cast<int64_t>(more);
+    if (need <= static_cast<int64_t>(A->pool_cap)) return BHX_OK;
+    if (static_cast<double>(need) * A->chunk_bytes() > static_cast<double>(kMaxPoolBytes) || need > 0xffffffffLL)
         return bhx_fail(ctx, BHX_E_TOO_LARGE, "checkpoint pool above 128 GiB: raise K");
     HIPCHK(hipDeviceSynchronize());
-    HIPCHK(A->pool.ensure(A->chunk_bytes() * static_cast<size_t>(taken)));
-    HIPCHK(A->link.ensure(sizeof(int32_t) * static_cast<size_t>(taken)));
-    A->pool_cap = taken;
+    HIPCHK(A->pool.ensure(A->chunk_bytes() * static_cast<size_t>(need)));
+    HIPCHK(A->link.ensure(sizeof(int32_t) * static_cast<size_t>(need)));
+    A->pool_cap = static_cast<uint32_t>(need);
     return adj_record(A, nullptr, st);
 }
 
@@ -489,7 +530,13 @@ int adj_set_K(bhx_adjoint* A, int K) {
     HIPCHK(A->seg.ensure(sizeof(double) * kSeg * (K + 1) * lanes));
     A->K = K;
     A->ck_lg = chunk_lg(K);
-    A->pool_cap = static_cast<uint32_t>(std::min<size_t>(A->pool.bytes / A->chunk_bytes(), A->link.bytes / 4));
+    // at least a first chunk per ray
+    if (static_cast<double>(A->n_rays) * A->chunk_bytes() > static_cast<double>(kMaxPoolBytes))
+        return bhx_fail(ctx, BHX_E_TOO_LARGE, "checkpoint pool above 128 GiB: raise K");
+    HIPCHK(A->pool.ensure(A->chunk_bytes() * static_cast<size_t>(A->n_rays)));
+    HIPCHK(A->link.ensure(sizeof(int32_t) * static_cast<size_t>(A->n_rays)));
+    A->pool_cap = static_cast<uint32_t>(
+        std::min<size_t>({A->pool.bytes / A->chunk_bytes(), A->link.bytes / 4, size_t(0xffffffffu)}));
     return adj_set_scenes(A, A->scenes.data(), static_cast<int>(A->scenes.size()));
 }
 
@@ -535,7 +582,9 @@ int adj_create(bhx_context* ctx, const bhx_scene* s, int32_t n, int32_t K, bool 
     // grows the pool to the need
     const int64_t rec = (batch ? 256 : 512) / K + 1;
     const int64_t per_ray = (rec + (int64_t(1) << A->ck_lg) - 1) >> A->ck_lg;
-    const int64_t chunks = std::min<int64_t>(n_rays * per_ray, kMaxPoolBytes / static_cast<int64_t>(A->chunk_bytes()));
+    const int64_t max_chunks = std::min<int64_t>(kMaxPoolBytes / static_cast<int64_t>(A->chunk_bytes()), 0xffffffffLL);
+    if (n_rays > max_chunks) return bail(bhx_fail(ctx, BHX_E_TOO_LARGE, "checkpoint pool above 128 GiB: raise K"));
+    const int64_t chunks = std::min<int64_t>(n_rays * per_ray, max_chunks);
     A->pool_cap = static_cast<uint32_t>(chunks);
     if (A->cst.ensure(sizeof(AdjConst)) != hipSuccess ||
         (batch && A->cams.ensure(sizeof(FrameCam) * n) != hipSuccess) ||
@@ -719,6 +768,16 @@ int bhx_adjoint_autotune(bhx_adjoint* A, const double* w, const int32_t* Ks, int
 int bhx_adjoint_get_interval(const bhx_adjoint* A, int32_t* K) {
     if (!A || !K) return BHX_E_INVALID;
     *K = A->K;
+    return BHX_OK;
+}
+
+int bhx_adjoint_set_launch(bhx_adjoint* A, int32_t q_chunk, int32_t refill) {
+    if (!A) return BHX_E_INVALID;
+    bhx_context* ctx = A->ctx;
+    if (q_chunk < 0 || q_chunk > 4096) return bhx_fail(ctx, BHX_E_INVALID, "queue chunk must be in [0, 4096]");
+    if (refill < 0 || refill > 64) return bhx_fail(ctx, BHX_E_INVALID, "refill must be in [0, 64]");
+    if (q_chunk) A->q_chunk = q_chunk;
+    if (refill) A->refill = refill;
     return BHX_OK;
 }
 

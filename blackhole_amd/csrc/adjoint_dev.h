@@ -998,11 +998,23 @@ __host__ __device__ inline int bwd_ray(const Frame& F, const FrameCam& cam, int6
             replay(F, s, ck.get(j), n_from, n_to, seg, stride);
             if (!same_ck(s, ck.get(j + 1))) ++mismatch;
         }
+        // the stored step i - 1 is loaded while step i is swept (one wave per
+        // SIMD hides no load latency by itself)
+        double nx[kSeg];
+        {
+            const double* o = seg + static_cast<int64_t>(n_to - n_from - 1) * kSeg * stride;
+            for (int c = 0; c < kSeg; ++c) nx[c] = o[c * stride];
+        }
         for (int i = n_to - n_from - 1; i >= 0; --i) {
-            const double* o = seg + static_cast<int64_t>(i) * kSeg * stride;
-            const double xi[4] = {o[0], o[stride], o[2 * stride], o[3 * stride]};
+            const double xi[4] = {nx[0], nx[1], nx[2], nx[3]};
+            const double hi = nx[4];
+            const bool jet_i = nx[5] != 0.0;
+            if (i > 0) {
+                const double* o = seg + static_cast<int64_t>(i - 1) * kSeg * stride;
+                for (int c = 0; c < kSeg; ++c) nx[c] = o[c * stride];
+            }
             const bool term = disc && j == j_last && i == n_to - n_from - 1;
-            step_reverse(F, K, xi, o[4 * stride], o[5 * stride] != 0.0, term, se, Or, wk, xb, cb, &J, lds);
+            step_reverse(F, K, xi, hi, jet_i, term, se, Or, wk, xb, cb, &J, lds);
         }
     }
     // camera: x0 = (r_cam, theta_cam, p_r, p_theta) and (p_t, p_phi) as

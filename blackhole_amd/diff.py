@@ -126,6 +126,11 @@ class DiscRender:
         self.generation += 1
         return k.value, ms.value
 
+    def set_launch(self, q_chunk=0, refill=0):
+        """The recording forward's queue chunk and refill threshold (0 keeps a value)."""
+        native.check(self.tr.ctx, self.lib.bhx_adjoint_set_launch(self.adj, int(q_chunk), int(refill)),
+                     "bhx_adjoint_set_launch")
+
     def info(self):
         ck, disc, mism = C.c_int64(), C.c_int64(), C.c_int64()
         f, b = C.c_double(), C.c_double()
@@ -205,6 +210,7 @@ class DiscBatch:
         return out
 
     autotune = DiscRender.autotune
+    set_launch = DiscRender.set_launch
     info = DiscRender.info
 
 
@@ -242,17 +248,25 @@ class _DiscImage(torch.autograd.Function):
 
 def gaussian_blur(img, sigma=2.0):
     """Separable Gaussian of radius ceil(3 sigma), clamp-to-edge, normalised
-    weights - the blur of img::add_glare (src/image.cpp:61-93) - on a 2-D
-    tensor, differentiable."""
+    weights - the blur of img::add_glare (src/image.cpp:61-93) - over the
+    last two axes of a (..., H, W) tensor, differentiable."""
     r = max(1, int(math.ceil(3.0 * sigma)))
-    i = torch.arange(-r, r + 1, device=img.device)
-    k = torch.exp(-0.5 * (i.to(torch.float64) ** 2) / (sigma * sigma))
-    k = k / k.sum()
-    H, W = img.shape
-    ix = (torch.arange(W, device=img.device)[:, None] + i[None, :]).clamp(0, W - 1)
-    tmp = (img[:, ix] * k).sum(-1)
-    iy = (torch.arange(H, device=img.device)[:, None] + i[None, :]).clamp(0, H - 1)
-    return (tmp[iy, :] * k[None, :, None]).sum(1)
+    i = torch.arange(-r, r + 1, dtype=torch.float64)
+    k = torch.exp(-0.5 * (i ** 2) / (sigma * sigma))
+    k = (k / k.sum()).tolist()
+    H, W = img.shape[-2:]
+    # clamp-to-edge: the border rows/columns repeated r times; each pass a
+    # sum of shifted slices (no gathers, so no scattered adds backward)
+    xp = torch.cat([img[..., :1].expand(*img.shape[:-1], r), img, img[..., -1:].expand(*img.shape[:-1], r)], -1)
+    tmp = k[0] * xp[..., 0:W]
+    for t in range(1, 2 * r + 1):
+        tmp = tmp + k[t] * xp[..., t:t + W]
+    lead = tmp.shape[:-2]
+    tp = torch.cat([tmp[..., :1, :].expand(*lead, r, W), tmp, tmp[..., -1:, :].expand(*lead, r, W)], -2)
+    out = k[0] * tp[..., 0:H, :]
+    for t in range(1, 2 * r + 1):
+        out = out + k[t] * tp[..., t:t + H, :]
+    return out
 
 
 def gaussian_blur_np(img, sigma=2.0):

@@ -318,9 +318,10 @@ int bhx_batch_autotune(bhx_batch *batch, int32_t reps, bhx_launch *best, double 
  * dL/dI per sample, the gradient of L with respect to (spin, cam_theta [rad],
  * eddington, cam_r).  The forward records a checkpoint of the integrator
  * state every K accepted steps (64 bytes each) into a device pool of
- * chunks that each ray takes as it goes (16 records for K < 16, 8 for
- * K < 64, else 4), so the pool holds about sum over rays of
- * (steps / K + 1) x 64 bytes, whatever max_steps is; a
+ * chunks (16 records for K < 16, 8 for K < 64, else 4): one per ray to
+ * start with and more as a ray needs them, so the pool holds about sum over
+ * rays of (steps / K + 1) x 64 bytes rounded up to chunks, whatever
+ * max_steps is; a
  * forward that finds the pool short still completes its image, and the next
  * backward grows the pool to the forward's count and records the same
  * forward again before it sweeps; the backward replays each K-step segment from its checkpoint with
@@ -360,6 +361,11 @@ int bhx_adjoint_autotune(bhx_adjoint *adj, const double *w, const int32_t *Ks, i
                          int32_t *best_K, double *best_ms, void *stream);
 /* The checkpoint interval in use. */
 int bhx_adjoint_get_interval(const bhx_adjoint *adj, int32_t *K);
+/* The recording forward's queue: a wave takes q_chunk consecutive slots at a
+ * time and refills its idle lanes once `refill` of them wait (a ray's set-up
+ * then runs on many lanes at once); 0 keeps a value.  Defaults 32 and 4.
+ * Results do not depend on them. */
+int bhx_adjoint_set_launch(bhx_adjoint *adj, int32_t q_chunk, int32_t refill);
 /* A batch of n frames of one size and one physics (spin, disc, jet,
  * integrator, cam_r) that differ in their camera only (cam_theta, cam_phi,
  * fov, roll, jitter seed): one persistent forward over all n x samples rays,
