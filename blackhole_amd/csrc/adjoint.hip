@@ -350,10 +350,11 @@ struct bhx_adjoint {
         partial, counters, qmap, contrib, frame_grad;
     uint32_t pool_cap = 0;           // chunks in the checkpoint pool (>= n_rays: a first chunk per ray)
     int32_t q_chunk = 32;            // the forward's queue: slots per wave take
-    int32_t refill = 4;              // idle lanes a wave waits for before it refills them
+    int32_t refill = 1;              // idle lanes a wave waits for before it refills them
     int grid_f = 0, grid_b = 0;
     double fwd_ms = 0.0, bwd_ms = 0.0;
     bool fwd_pending = false;
+    bool have_steps = false;         // steps holds a forward's step counts (the next queue order)
     unsigned int mismatches = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     size_t chunk_bytes() const { return sizeof(double) * kCk * (size_t(1) << ck_lg); }
@@ -419,6 +420,17 @@ int adj_record(bhx_adjoint* A, double* I, hipStream_t st) {
     P.n_lit = reinterpret_cast<unsigned int*>(static_cast<char*>(A->counters.p) + 8);
     P.cams = static_cast<const FrameCam*>(A->cams.p);
     P.n_per = A->n_per;
+    // the queue: after a first forward in the frames' tile order, the rays by
+    // the last forward's accepted steps, most first, so that the longest rays
+    // start first and no lane is left with one at the end (the order changes
+    // which lane traces a ray, never what the ray computes)
+    if (A->have_steps) {
+        size_t tq = A->sort_tmp.bytes;
+        HIPCHK(hipcub::DeviceRadixSort::SortPairsDescending(A->sort_tmp.p, tq, static_cast<const int32_t*>(A->steps.p),
+                                                            static_cast<int32_t*>(A->key_sorted.p),
+                                                            static_cast<const int32_t*>(A->iota.p),
+                                                            static_cast<int32_t*>(A->qmap.p), A->n_rays, 0, 31, st));
+    }
     // counters: [0] queue, [8] lit rays, [16] pool chunks taken, [32] replay mismatches
     HIPCHK(launch_zero_u64(static_cast<unsigned long long*>(A->counters.p), 8, st));
     HIPCHK(hipEventRecord(A->ev[0], st));
@@ -438,6 +450,7 @@ int adj_record(bhx_adjoint* A, double* I, hipStream_t st) {
                                                         static_cast<int32_t*>(A->order.p), A->n_rays, 0, 15, st));
     HIPCHK(hipEventRecord(A->ev[1], st));
     A->fwd_pending = true;
+    A->have_steps = true;
     return BHX_OK;
 }
 
@@ -559,6 +572,10 @@ int adj_create(bhx_context* ctx, const bhx_scene* s, int32_t n, int32_t K, bool 
     bhx_adjoint* A = new bhx_adjoint();
     A->ctx = ctx;
     A->is_batch = batch;
+    // a frame's rays fill the resident lanes a few times over: refill each
+    // idle lane at once; a batch's many frames run in steady state, where
+    // refilling 4 lanes at a time pays (measured on the 1024^2 sweep frames)
+    A->refill = batch ? 4 : 1;
     A->K = K;
     A->ck_lg = chunk_lg(K);
     A->n_per = n_per;
@@ -632,12 +649,16 @@ int adj_create(bhx_context* ctx, const bhx_scene* s, int32_t n, int32_t K, bool 
                        static_cast<int32_t*>(A->iota.p), n_rays);
     if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
         return bail(bhx_fail(ctx, BHX_E_HIP, "k_iota"));
-    size_t tmp = 0;
+    size_t tmp = 0, tmp31 = 0;
     if (hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp, static_cast<const int32_t*>(A->key.p),
                                                      static_cast<int32_t*>(A->key_sorted.p),
                                                      static_cast<const int32_t*>(A->iota.p),
                                                      static_cast<int32_t*>(A->order.p), n_rays, 0, 15) != hipSuccess ||
-        A->sort_tmp.ensure(std::max<size_t>(tmp, 16)) != hipSuccess)
+        hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp31, static_cast<const int32_t*>(A->steps.p),
+                                                     static_cast<int32_t*>(A->key_sorted.p),
+                                                     static_cast<const int32_t*>(A->iota.p),
+                                                     static_cast<int32_t*>(A->qmap.p), n_rays, 0, 31) != hipSuccess ||
+        A->sort_tmp.ensure(std::max<size_t>({tmp, tmp31, size_t(16)})) != hipSuccess)
         return bail(bhx_fail(ctx, BHX_E_NOMEM, "radix sort scratch"));
     const int rc = adj_set_scenes(A, s, n);
     if (rc) return bail(rc);

@@ -363,7 +363,8 @@ int bhx_adjoint_autotune(bhx_adjoint *adj, const double *w, const int32_t *Ks, i
 int bhx_adjoint_get_interval(const bhx_adjoint *adj, int32_t *K);
 /* The recording forward's queue: a wave takes q_chunk consecutive slots at a
  * time and refills its idle lanes once `refill` of them wait (a ray's set-up
- * then runs on many lanes at once); 0 keeps a value.  Defaults 32 and 4.
+ * then runs on many lanes at once); 0 keeps a value.  Defaults: 32 and 1
+ * (a single frame), 32 and 4 (a batch).
  * Results do not depend on them. */
 int bhx_adjoint_set_launch(bhx_adjoint *adj, int32_t q_chunk, int32_t refill);
 /* A batch of n frames of one size and one physics (spin, disc, jet,

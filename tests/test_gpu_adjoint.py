@@ -320,3 +320,26 @@ def test_batch_refusals(tracer):
     assert tracer.lib.bhx_adjoint_backward(db.adj, C.c_void_p(w.data_ptr()), g, None) != 0
     assert np.all(db.backward_raw(w) == 0.0)
     db.close()
+
+
+def test_forward_queue_shape_leaves_results_unchanged(tracer):
+    """The recording forward's queue chunk and refill threshold change the
+    order lanes take rays in, not what any ray computes: intensities and
+    gradients bit for bit the same; out-of-range shapes are refused."""
+    import torch
+    from blackhole_amd import BhxError
+    sc = _scene(48)
+    dr = DiscRender(tracer, sc, K=16)
+    I1 = dr.forward_raw(P0).clone()
+    w = torch.linspace(-1.0, 1.0, sc.n_rays, dtype=torch.float64, device=I1.device)
+    g1 = dr.backward_raw(w)
+    for q, r in ((1, 1), (7, 3), (64, 64), (4096, 2)):
+        dr.set_launch(q, r)
+        I2 = dr.forward_raw(P0)
+        assert torch.equal(I1, I2), (q, r)
+        assert np.array_equal(dr.backward_raw(w), g1), (q, r)
+    with pytest.raises(BhxError):
+        dr.set_launch(5000, 0)
+    with pytest.raises(BhxError):
+        dr.set_launch(0, 65)
+    dr.close()
