@@ -18,6 +18,10 @@ HIP_SOURCES = ["tracer.hip", "shade.hip", "api.hip", "order.hip", "batch.hip", "
 # tableau, sincos/log/exp coefficients); FP64 VALU ops take no 64-bit
 # literals, so machine LICM would pin every one of them in two registers for
 # the whole loop and spill.  Without it they are rematerialised where used.
+# tracer.hip is compiled with -ffp-contract=on as well: its kernels (single
+# scene, lean, batched, pilot) inline one tracer body into differently shaped
+# code, and with FMAs formed per source expression they round identically, so
+# a ray's bits do not depend on which of them traced it.
 # adjoint.hip: the recording forward and the backward's replay inline the same
 # DP5 attempt into two kernels and must round identically, so FMAs are formed
 # per source expression only (-ffp-contract=on), never across expressions by
@@ -26,7 +30,7 @@ HIP_SOURCES = ["tracer.hip", "shade.hip", "api.hip", "order.hip", "batch.hip", "
 # shade.hip: the same LICM setting as the tracer's (the shading loop inlines the
 # library's acos / atan2 / exp / log / pow, whose constants would otherwise be
 # hoisted into registers: 256 VGPRs and scratch spills instead of 146 and none).
-EXTRA = {"tracer.hip": ["-mllvm", "-disable-machine-licm"],
+EXTRA = {"tracer.hip": ["-mllvm", "-disable-machine-licm", "-ffp-contract=on"],
          "noalloc.hip": ["-ffp-contract=off"],
          "shade.hip": ["-mllvm", "-disable-machine-licm"],
          "adjoint.hip": ["-mllvm", "-disable-machine-licm", "-ffp-contract=on"],

@@ -95,10 +95,10 @@ int ensure_catalogue(bhx_context* ctx, const bhx_scene* s) {
     return BHX_OK;
 }
 
-int trace_grid(bhx_context* ctx, int* grid, int* block) {
+int trace_grid(bhx_context* ctx, bool lean, int* grid, int* block) {
     *block = ctx->launch.block > 0 ? ctx->launch.block : 256;
     int per_cu = 0;
-    HIPCHK(trace_occupancy(*block, &per_cu));
+    HIPCHK(trace_occupancy(*block, lean, &per_cu));
     if (per_cu <= 0) return fail(ctx, BHX_E_HIP, "tracer kernel cannot be resident");
     if (ctx->launch.waves_per_simd > 0) {
         const int want = std::max(1, ctx->launch.waves_per_simd * 4 * 64 / *block);
@@ -266,7 +266,7 @@ int bhx_trace_dev(bhx_context* ctx, const bhx_scene* s, const bhx_rays* rays, bh
     P.carter = rays->carter;
     P.totals = ctx->d_cnt->totals;
     int grid = 0, block = 0;
-    rc = trace_grid(ctx, &grid, &block);
+    rc = trace_grid(ctx, trace_is_lean(P), &grid, &block);
     if (rc) return rc;
     // counters are cleared by a fill kernel (never a memset node: the call may be captured)
     HIPCHK(launch_zero_u64(reinterpret_cast<unsigned long long*>(ctx->d_cnt), sizeof(Counters) / 8, st));

@@ -15,7 +15,10 @@
 namespace bhx {
 hipError_t launch_raygen(const RaygenArgs& args, hipStream_t stream);
 hipError_t launch_trace(const TraceArgs& args, int grid, int block, hipStream_t stream);
-hipError_t trace_occupancy(int block, int* blocks_per_cu);
+// Tracer blocks resident per CU, for the lean kernel (scenes without jets or
+// the Sun, three waves per SIMD) or the full one (two).
+hipError_t trace_occupancy(int block, bool lean, int* blocks_per_cu);
+bool trace_is_lean(const TraceArgs& args);
 hipError_t launch_pilot(const RaygenArgs& gen, const TraceArgs& args, int block, hipStream_t stream);
 hipError_t launch_trace_batch(const TraceArgs& args, int grid, int block, hipStream_t stream);
 hipError_t trace_batch_occupancy(int block, int n_scenes, int* blocks_per_cu);

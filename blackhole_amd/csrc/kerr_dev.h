@@ -55,7 +55,11 @@ __host__ __device__ __forceinline__ double rcp_coarse(double x) {
 // for the library routine whose Payne-Hanek branch would split every RHS into
 // separate basic blocks.  theta cannot leave that range: each DP5 step moves it
 // by less than pi and a ray takes at most 20000 steps.  NaN stays NaN.
-__host__ __device__ __forceinline__ void sincos_near(double x, double* s, double* c) {
+// sincos_parts stops before the quadrant's common sign: x = r + q pi/2 gives
+// (sin x, cos x) = (a, b), both signs flipped when bit 1 of q is set; it
+// returns a, b and that flip as a sign-bit mask.  A caller that needs only
+// sin^2, cos^2 and sin x cos x (the flips cancel in the product) skips it.
+__host__ __device__ __forceinline__ void sincos_parts(double x, double* sa, double* cb, uint64_t* flip) {
     const double k = rint(x * 0.63661977236758134308);
     double r = fma(-k, 1.5707963267948966, x);
     r = fma(-k, 6.123233995736766e-17, r);
@@ -68,18 +72,42 @@ __host__ __device__ __forceinline__ void sincos_near(double x, double* s, double
         2.08757232129817482790e-09), -2.75573143513906633035e-07), 2.48015872894767294178e-05),
         -1.38888888888741095749e-03), 4.16666666666666019037e-02);
     const double hz = 0.5 * z;
-    const double w = 1.0 - hz;
+    const double w = fma(-0.5, z, 1.0);   // 1 - hz, hz exact
     const double cr = w + (((1.0 - w) - hz) + z * z * pc);
     // x = r + q pi/2: (sin x, cos x) = (sr, cr), (cr, -sr), (-sr, -cr), (-cr, sr)
-    // for q = 0..3 (mod 4): pick by the low bit, then flip both signs by bit 1
-    // (an exclusive or of the sign bit, no selects)
+    // for q = 0..3 (mod 4): pick by the low bit; bit 1 flips both signs
     const int q = static_cast<int>(k);
     const bool odd = (q & 1) != 0;
-    const double a = odd ? cr : sr;
-    const double b = odd ? -sr : cr;
-    const uint64_t flip = static_cast<uint64_t>(q & 2) << 62;
-    *s = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, a) ^ flip);
-    *c = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, b) ^ flip);
+    *sa = odd ? cr : sr;
+    *cb = odd ? -sr : cr;
+    *flip = static_cast<uint64_t>(q & 2) << 62;
+}
+
+// Flip the sign of x where `flip` has the sign bit set (an exclusive or).
+__host__ __device__ __forceinline__ double flip_sign(double x, uint64_t flip) {
+    return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, x) ^ flip);
+}
+
+__host__ __device__ __forceinline__ void sincos_near(double x, double* s, double* c) {
+    double a, b;
+    uint64_t flip;
+    sincos_parts(x, &a, &b, &flip);
+    *s = flip_sign(a, flip);
+    *c = flip_sign(b, flip);
+}
+
+// fma(a, b, c) for a constant c: on the device one v_fma_f64 taking c from a
+// scalar register pair (the backend would otherwise copy c into vector
+// registers to use the accumulating form, two extra vector moves per term of
+// a Horner evaluation).
+__host__ __device__ __forceinline__ double fma_k(double a, double b, double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double d;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
+    return d;
+#else
+    return fma(a, b, c);
+#endif
 }
 
 // exp(x) for |x| < 700 to <= 1 ulp (checked against glibc): Cody-Waite
@@ -89,17 +117,16 @@ __host__ __device__ __forceinline__ double exp_near(double x) {
     const double k = rint(x * 1.4426950408889634);
     double r = fma(-k, 6.93147180369123816490e-01, x);
     r = fma(-k, 1.90821492927058770002e-10, r);
-    double p = 1.0 / 6227020800.0;
-    p = fma(p, r, 1.0 / 479001600.0);
-    p = fma(p, r, 1.0 / 39916800.0);
-    p = fma(p, r, 1.0 / 3628800.0);
-    p = fma(p, r, 1.0 / 362880.0);
-    p = fma(p, r, 1.0 / 40320.0);
-    p = fma(p, r, 1.0 / 5040.0);
-    p = fma(p, r, 1.0 / 720.0);
-    p = fma(p, r, 1.0 / 120.0);
-    p = fma(p, r, 1.0 / 24.0);
-    p = fma(p, r, 1.0 / 6.0);
+    double p = fma_k(1.0 / 6227020800.0, r, 1.0 / 479001600.0);
+    p = fma_k(p, r, 1.0 / 39916800.0);
+    p = fma_k(p, r, 1.0 / 3628800.0);
+    p = fma_k(p, r, 1.0 / 362880.0);
+    p = fma_k(p, r, 1.0 / 40320.0);
+    p = fma_k(p, r, 1.0 / 5040.0);
+    p = fma_k(p, r, 1.0 / 720.0);
+    p = fma_k(p, r, 1.0 / 120.0);
+    p = fma_k(p, r, 1.0 / 24.0);
+    p = fma_k(p, r, 1.0 / 6.0);
     p = fma(p, r, 0.5);
     p = fma(p, r, 1.0);
     p = fma(p, r, 1.0);
@@ -115,10 +142,10 @@ __host__ __device__ __forceinline__ double log_pos(double x) {
     const double f = m - 1.0;
     const double s = f * rcp(2.0 + f);
     const double z = s * s, w = z * z;
-    const double t1 = w * (3.999999999940941908e-01 + w * (2.222219843214978396e-01 + w * 1.531383769920937332e-01));
-    const double t2 = z * (6.666666666666735130e-01 + w * (2.857142874366239149e-01 +
-                           w * (1.818357216161805012e-01 + w * 1.479819860511658591e-01)));
-    const double R = t2 + t1;
+    const double t2 = z * fma_k(w, fma_k(w, fma_k(w, 1.479819860511658591e-01, 1.818357216161805012e-01),
+                                         2.857142874366239149e-01), 6.666666666666735130e-01);
+    const double R = fma(w, fma_k(w, fma_k(w, 1.531383769920937332e-01, 2.222219843214978396e-01),
+                                  3.999999999940941908e-01), t2);   // t2 + t1
     const double hfsq = 0.5 * f * f;
     const double dk = e;
     return dk * 6.93147180369123816490e-01 - ((hfsq - (s * (hfsq + R) + dk * 1.90821492927058770002e-10)) - f);
@@ -171,13 +198,14 @@ __host__ __device__ __forceinline__ void kerr_rhs_sc(const RayK& K, double r, do
     sn = copysign(fmax(fabs(sn), 1e-9), sn);
     const double sn2 = sn * sn;
     const double r2 = r * r;
-    const double rpa = r2 + K.a2;
+    const double rpa = fma(r, r, K.a2);
     const double S = r2 + K.a2 * (cs * cs);
     const double D = rpa - 2.0 * r;
     const double SD = S * D;
     const double iX = rcp(SD * sn2);          // 1 / (Sigma Delta sin^2)
-    const double iS = D * sn2 * iX;
-    const double iD = S * sn2 * iX;
+    const double iSD = sn2 * iX;              // 1 / (Sigma Delta)
+    const double iS = D * iSD;
+    const double iD = S * iSD;
     const double isn2 = SD * iX;
     const double P = rpa * K.pt + K.apph;
     const double PiD = P * iD;
@@ -203,11 +231,17 @@ __host__ __device__ __forceinline__ void kerr_rhs_sc(const RayK& K, double r, do
     o.rpa = rpa;
 }
 
+// The right-hand side depends on sin theta and cos theta only through sin^2,
+// cos^2 and sin cos, in which the quadrant's common sign flip cancels: the
+// unflipped pair goes in, and only the cosine kept for the event tests (o.cs)
+// gets its sign - the same bits as with the flipped pair.
 __host__ __device__ __forceinline__ void kerr_rhs(const RayK& K, double r, double th, double pr, double pth,
                                          Rhs& o) {
-    double sn, cs;
-    sincos_near(th, &sn, &cs);
-    kerr_rhs_sc(K, r, sn, cs, pr, pth, o);
+    double sa, cb;
+    uint64_t flip;
+    sincos_parts(th, &sa, &cb, &flip);
+    kerr_rhs_sc(K, r, sa, cb, pr, pth, o);
+    o.cs = flip_sign(cb, flip);
 }
 
 // splitmix64 (the reference's Rng, src/scene.h:27-44), counter form: draw n

@@ -61,7 +61,8 @@ struct DeviceState {
     std::once_flag once;
     int rc = BHX_E_HIP;
     unsigned char* base = nullptr;   // device address of scratch on this GPU
-    int grid = 0;
+    int grid = 0;        // persistent grid of the full tracer kernel
+    int grid_lean = 0;   // and of the lean one (scenes without jets or the Sun)
 };
 DeviceState g_device[kMaxDevices];
 
@@ -75,11 +76,13 @@ int fail(int code, const char* msg) {
 void set_up(int device, DeviceState& d) {
     void* p = nullptr;
     if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return;
-    int n_cu = 0, per_cu = 0;
+    int n_cu = 0, per_cu = 0, per_cu_lean = 0;
     if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return;
-    if (trace_occupancy(kBlock, &per_cu) != hipSuccess || per_cu <= 0 || n_cu <= 0) return;
+    if (trace_occupancy(kBlock, false, &per_cu) != hipSuccess || per_cu <= 0 || n_cu <= 0) return;
+    if (trace_occupancy(kBlock, true, &per_cu_lean) != hipSuccess || per_cu_lean <= 0) return;
     d.base = static_cast<unsigned char*>(p);
     d.grid = per_cu * n_cu;
+    d.grid_lean = per_cu_lean * n_cu;
     d.rc = BHX_OK;
 }
 
@@ -212,7 +215,8 @@ extern "C" int bhx_trace_noalloc_dev(int32_t device, const bhx_scene* s, const b
     if (launch_zero_u64(reinterpret_cast<unsigned long long*>(cnt), sizeof(Counters) / 8, st) != hipSuccess)
         return fail(BHX_E_HIP, "counter clear failed");
     if (launch_raygen(G, st) != hipSuccess) return fail(BHX_E_HIP, "ray generation launch failed");
-    if (launch_trace(P, d.grid, kBlock, st) != hipSuccess) return fail(BHX_E_HIP, "tracer launch failed");
+    if (launch_trace(P, trace_is_lean(P) ? d.grid_lean : d.grid, kBlock, st) != hipSuccess)
+        return fail(BHX_E_HIP, "tracer launch failed");
     if (stats) {
         Counters c;
         if (hipStreamSynchronize(st) != hipSuccess ||

@@ -125,10 +125,15 @@ __global__ void __launch_bounds__(256) k_pilot_gen(const RaygenArgs P) {
 // scene constant is a kernel argument, uniform, in scalar registers) and the
 // batched kernel (kBatch = true: each lane's ray belongs to its own scene; the
 // spin and tolerances live in the lane's registers, everything else in the
-// LDS copy `sk` of the scene table).
+// LDS copy `sk` of the scene table).  kLean drops the jet and Sun branches at
+// compile time (scenes of disc, neutron star, horizon and sky only): the
+// kernel then holds its state in 168 VGPRs and runs three waves per SIMD
+// instead of two; the arithmetic of every branch it keeps is unchanged.
 #define SK(f) (kBatch ? sk[sid].f : P.f)
 #define EV (kBatch ? sk[sid].ev : *P.ev)
-template <bool kBatch, bool kPilot = false>
+#define HAS_JET (!kLean && SK(has_jet))
+#define HAS_SUN (!kLean && SK(has_sun))
+template <bool kBatch, bool kPilot = false, bool kLean = false>
 __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk, double* pend,
                                            unsigned long long* blk) {
     // ---- per-scene constants -------------------------------------------------
@@ -140,6 +145,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
     int q = 0;                          // queue slot of the lane's ray
     double y[6], k1[6];                 // t, r, th, ph, p_r, p_th
     double pt = 0.0, pph = 0.0;         // conserved p_t, p_phi
+    double kapt = 0.0, kapph = 0.0, ka2pt2 = 0.0, kpph2 = 0.0;   // their products in RayK
     double h = 0.0, lpw = 0.0, c_prev = 0.0, q0raw = 0.0, iq0 = 0.0;   // lpw = beta log(err_prev)
     double mx_norm = 0.0, mx_carter = 0.0, jet_acc = 0.0;
     int nsteps = 0, natt = 0, tries = 0;
@@ -158,7 +164,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
     // them from its LDS table.
     double ev_r_bound = 0.0, ev_cone_angle = 0.0, ev_sun_R = 0.0, ev_sun_c[3] = {0.0, 0.0, 0.0};
     int ev_model = 0;
-    if constexpr (!kBatch) {
+    if constexpr (!kBatch && !kLean) {
         if (P.has_jet || P.has_sun) {
             ev_r_bound = uniform_d(P.ev->r_bound);
             ev_cone_angle = uniform_d(P.ev->cone_angle);
@@ -296,6 +302,13 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
                     }
                     pt = I[kI_PT * N + q];
                     pph = I[kI_PPH * N + q];
+                    {
+                        const RayK K0 = ray_k(la, la2, pt, pph);
+                        kapt = K0.apt;
+                        kapph = K0.apph;
+                        ka2pt2 = K0.a2pt2;
+                        kpph2 = K0.pph2;
+                    }
                     c_prev = I[kI_C0 * N + q];
                     q0raw = I[kI_Q0 * N + q];
                     iq0 = rcp(fabs(q0raw) + 1e-12);
@@ -336,30 +349,40 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
             h = (tries == 0 && h > hmax) ? hmax : h;   // clamp once per step, not per retry
         }
         double tmp[6], k2[6], k3[6], k4[6], k5[6], k6[6], yn[6];
-        const RayK K = ray_k(la, la2, pt, pph);
+        RayK K;
+        K.a = la; K.a2 = la2; K.pt = pt; K.pph = pph;
+        K.apt = kapt; K.apph = kapph; K.a2pt2 = ka2pt2; K.pph2 = kpph2;
         Rhs f;
+        // stage inputs y + sum_j (h a_ij) k_j: the step size is folded into
+        // the tableau once per attempt (an FMA per term, no separate scaling);
+        // they only feed the right-hand side, the new state keeps y + h sum b_j k_j
+        const double h21 = h * A21;
 #pragma unroll
-        for (int c = 0; c < 6; ++c) tmp[c] = y[c] + h * A21 * k1[c];
+        for (int c = 0; c < 6; ++c) tmp[c] = fma(h21, k1[c], y[c]);
         kerr_rhs(K, tmp[1], tmp[2], tmp[4], tmp[5], f);
+        const double h31 = h * A31, h32 = h * A32;
 #pragma unroll
-        for (int c = 0; c < 6; ++c) { k2[c] = f.d[c]; tmp[c] = y[c] + h * (A31 * k1[c] + A32 * k2[c]); }
+        for (int c = 0; c < 6; ++c) { k2[c] = f.d[c]; tmp[c] = fma(h32, k2[c], fma(h31, k1[c], y[c])); }
         kerr_rhs(K, tmp[1], tmp[2], tmp[4], tmp[5], f);
+        const double h41 = h * A41, h42 = h * A42, h43 = h * A43;
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             k3[c] = f.d[c];
-            tmp[c] = y[c] + h * (A41 * k1[c] + A42 * k2[c] + A43 * k3[c]);
+            tmp[c] = fma(h43, k3[c], fma(h42, k2[c], fma(h41, k1[c], y[c])));
         }
         kerr_rhs(K, tmp[1], tmp[2], tmp[4], tmp[5], f);
+        const double h51 = h * A51, h52 = h * A52, h53 = h * A53, h54 = h * A54;
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             k4[c] = f.d[c];
-            tmp[c] = y[c] + h * (A51 * k1[c] + A52 * k2[c] + A53 * k3[c] + A54 * k4[c]);
+            tmp[c] = fma(h54, k4[c], fma(h53, k3[c], fma(h52, k2[c], fma(h51, k1[c], y[c]))));
         }
         kerr_rhs(K, tmp[1], tmp[2], tmp[4], tmp[5], f);
+        const double h61 = h * A61, h62 = h * A62, h63 = h * A63, h64 = h * A64, h65 = h * A65;
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             k5[c] = f.d[c];
-            tmp[c] = y[c] + h * (A61 * k1[c] + A62 * k2[c] + A63 * k3[c] + A64 * k4[c] + A65 * k5[c]);
+            tmp[c] = fma(h65, k5[c], fma(h64, k4[c], fma(h63, k3[c], fma(h62, k2[c], fma(h61, k1[c], y[c])))));
         }
         kerr_rhs(K, tmp[1], tmp[2], tmp[4], tmp[5], f);
 #pragma unroll
@@ -423,21 +446,35 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
             }
             const bool ns_hit = SK(has_ns) && yn[1] <= SK(ns_R);
             const bool horizon = !ns_hit && yn[1] <= SK(r_stop);
-            const bool disc_cross = !ns_hit && !horizon && SK(has_disc) && c_prev * g7.cs < 0.0;
+            bool disc_cross = !ns_hit && !horizon && SK(has_disc) && c_prev * g7.cs < 0.0;
+            if (disc_cross) {
+                // Where the step's r(s) stays outside [r_in, r_out] the crossing
+                // cannot hit the disc, and its root is not located: on [0, 1],
+                // r(s) - (c1 + s c2) = s(1-s)(c3 + s(c4 + (1-s) c5)) is at most
+                // |c3|/4 + 4|c4|/27 + |c5|/16, and c1 + s c2 lies between the
+                // end points.  The 1e-12 margins cover the rounding of the bound
+                // and of the root's own evaluation, so no hit is ever skipped.
+                const double c2r = yn[1] - y[1];
+                const double c3r = h * k1[1] - c2r;
+                const double c4r = c2r - h * g7.d[1] - c3r;
+                const double dev = 0.25 * fabs(c3r) + (4.0 / 27.0) * fabs(c4r) + 0.0625 * fabs(dense_c5(1));
+                const double lo = fmin(y[1], yn[1]) - dev, hi = fmax(y[1], yn[1]) + dev;
+                disc_cross = lo <= SK(r_out) * (1.0 + 1e-12) && hi >= SK(r_in) * (1.0 - 1e-12);
+            }
             // jet cull and Sun candidate need the pseudo-Cartesian end points of the step
             bool near_jet = false, sun_cand = false;
             double p0[3], p1[3];
-            if (SK(has_jet) || SK(has_sun)) {
+            if (HAS_JET || HAS_SUN) {
                 to_cart(la2, y[1], y[2], y[3], p0);
                 to_cart(la2, yn[1], yn[2], yn[3], p1);
-                if (SK(has_jet)) near_jet = seg_dist(p0, p1) < EVS(r_bound);        // render.cpp:151-152
+                if (HAS_JET) near_jet = seg_dist(p0, p1) < EVS(r_bound);        // render.cpp:151-152
                 // conical jet: only steps whose dense output can reach the cone
                 if (near_jet && EVS(model) == 1) {
                     const double c2t = yn[2] - y[2];
                     const double c3t = h * k1[2] - c2t;
                     near_jet = cone_reach(y[2], c2t, c3t, c2t - h * g7.d[2] - c3t, dense_c5(2), EVS(cone_angle));
                 }
-                if (SK(has_sun) && !ns_hit && !horizon) {                             // render.cpp:203-209
+                if (HAS_SUN && !ns_hit && !horizon) {                             // render.cpp:203-209
                     const double e0 = p1[0] - p0[0], e1 = p1[1] - p0[1], e2 = p1[2] - p0[2];
                     const double seg_len = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
                     const double f0 = p0[0] - EVS(sun_c)[0], f1 = p0[1] - EVS(sun_c)[1], f2 = p0[2] - EVS(sun_c)[2];
@@ -616,6 +653,8 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
 #undef SK
 #undef EV
 #undef EVS
+#undef HAS_JET
+#undef HAS_SUN
 
 __device__ __forceinline__ void flush_totals(const TraceArgs& P, unsigned long long* blk) {
     // frame totals: one global atomic per block
@@ -638,15 +677,26 @@ k_trace(const TraceArgs P) {
     flush_totals(P, blk);
 }
 
+// Scenes without jets or the Sun: the same tracer without those branches.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_trace_lean(const TraceArgs P) {
+    __shared__ unsigned long long blk[4];
+    if (threadIdx.x < 4) blk[threadIdx.x] = 0ull;
+    __syncthreads();
+    trace_body<false, false, true>(P, nullptr, nullptr, blk);
+    flush_totals(P, blk);
+}
+
 // Pilot of the cost-ordered queue: the same tracer with the coarse hmax, at
 // the tolerances and step cap its arguments carry.
+template <bool kLean>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 k_trace_pilot(const TraceArgs P) {
     __shared__ unsigned long long blk[4];
     extern __shared__ double pend_raw[];
     if (threadIdx.x < 4) blk[threadIdx.x] = 0ull;
     __syncthreads();
-    trace_body<false, true>(P, nullptr, pend_raw, blk);
+    trace_body<false, true, kLean>(P, nullptr, pend_raw, blk);
     flush_totals(P, blk);
 }
 
@@ -678,8 +728,14 @@ hipError_t launch_raygen(const RaygenArgs& args, hipStream_t stream) {
 // Dynamic LDS: [scene table (batched only)] [kPend doubles per lane].
 static size_t pend_bytes(int block) { return sizeof(double) * kPend * (size_t)block; }
 
+// The lean kernel serves every scene without jets or the Sun.
+bool trace_is_lean(const TraceArgs& args) { return !args.has_jet && !args.has_sun; }
+
 hipError_t launch_trace(const TraceArgs& args, int grid, int block, hipStream_t stream) {
-    hipLaunchKernelGGL(k_trace, dim3(grid), dim3(block), pend_bytes(block), stream, args);
+    if (trace_is_lean(args))
+        hipLaunchKernelGGL(k_trace_lean, dim3(grid), dim3(block), 0, stream, args);
+    else
+        hipLaunchKernelGGL(k_trace, dim3(grid), dim3(block), pend_bytes(block), stream, args);
     return hipGetLastError();
 }
 
@@ -690,7 +746,10 @@ hipError_t launch_pilot(const RaygenArgs& gen, const TraceArgs& args, int block,
     // every wave takes one chunk of 64 pilot rays: n_tiles / 64 waves in all
     const int waves = (gen.n_tiles + 63) / 64;
     const int grid = (waves * 64 + block - 1) / block;
-    hipLaunchKernelGGL(k_trace_pilot, dim3(grid), dim3(block), pend_bytes(block), stream, args);
+    if (trace_is_lean(args))
+        hipLaunchKernelGGL(k_trace_pilot<true>, dim3(grid), dim3(block), 0, stream, args);
+    else
+        hipLaunchKernelGGL(k_trace_pilot<false>, dim3(grid), dim3(block), pend_bytes(block), stream, args);
     return hipGetLastError();
 }
 
@@ -700,7 +759,8 @@ hipError_t launch_trace_batch(const TraceArgs& args, int grid, int block, hipStr
     return hipGetLastError();
 }
 
-hipError_t trace_occupancy(int block, int* blocks_per_cu) {
+hipError_t trace_occupancy(int block, bool lean, int* blocks_per_cu) {
+    if (lean) return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_trace_lean, block, 0);
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_trace, block, pend_bytes(block));
 }
 

@@ -16,9 +16,16 @@ The bar (one ray = one sample):
     along a ray both are carried unchanged, so their drift is exactly zero in
     both implementations.
 "Within EPS of a class boundary" is made operational: the oracle re-traces the
-ray at image-plane offsets (u +- EPS, v) and (u, v +- EPS), u, v in [-1, 1]
-(src/render.cpp:447-448); if any of those four rays ends in another class
-than the oracle's own, the ray is exempt from the class check.
+ray at image-plane offsets (u +- d, v) and (u, v +- d), u, v in [-1, 1]
+(src/render.cpp:447-448), for d = EPS, EPS/10, EPS/100 and EPS/1000; if any
+of those rays ends in another class than the oracle's own, the ray is exempt
+from the class check, and the value spread is the largest over all of them.
+The smaller offsets sample the EPS neighbourhood where the result is not a
+smooth function of the ray: a ray that grazes a pole of the Boyer-Lindquist
+coordinates, where the reference clamps sin(theta) at 1e-9 (src/kerr.h:34),
+can change its accepted-step count under a 1e-11 displacement, and with it
+its escape direction by ~1e-8, while the four rays at exactly +-EPS happen to
+land on the same side of that jump.
 """
 import numpy as np
 
@@ -58,22 +65,24 @@ def compare(scene, ids, gpu, cpu, cat=None, rel=REL, eps=EPS, threads=0):
     exempt = np.zeros(n, bool)
     if len(suspects):
         u, v = oracle.ray_uv(scene, ids[suspects])
-        du = np.array([eps, -eps, 0.0, 0.0])
-        dv = np.array([0.0, 0.0, eps, -eps])
+        scales = eps * np.array([1.0, 1e-1, 1e-2, 1e-3])
+        du = np.concatenate([[d, -d, 0.0, 0.0] for d in scales])
+        dv = np.concatenate([[0.0, 0.0, d, -d] for d in scales])
+        m = len(du)
         U = (u[:, None] + du[None, :]).ravel()
         V = (v[:, None] + dv[None, :]).ravel()
         probe = oracle.trace_uv(scene, U, V, cat=cat, threads=threads)
-        pc = probe["cls"].reshape(-1, 4).astype(int)
+        pc = probe["cls"].reshape(-1, m).astype(int)
         centre = cc[suspects]
         boundary = (pc != centre[:, None]).any(axis=1)
-        # conditioning of the value itself under the same displacement
-        pr = probe["y"][1].reshape(-1, 4)
+        # conditioning of the value itself under the same displacements
+        pr = probe["y"][1].reshape(-1, m)
         spread_r = np.where(np.isin(centre, (1, 2, 3)), np.abs(pr - cpu["y"][1][suspects, None]).max(axis=1) / np.maximum(
             np.abs(cpu["y"][1][suspects]), 1e-300), 0.0)
-        pg = probe["g"].reshape(-1, 4)
+        pg = probe["g"].reshape(-1, m)
         spread_g = np.abs(pg - cpu["g"][suspects, None]).max(axis=1) / np.maximum(
             np.abs(cpu["g"][suspects]), 1e-300)
-        pdir = probe["dir"].reshape(3, -1, 4)
+        pdir = probe["dir"].reshape(3, -1, m)
         spread_d = np.linalg.norm(pdir - cpu["dir"][:, suspects, None], axis=0).max(axis=1)
         spread = np.maximum(np.maximum(spread_r, np.where(cpu["g"][suspects] != 0, spread_g, 0)),
                             np.where(centre == 4, spread_d, 0))
