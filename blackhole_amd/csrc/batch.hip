@@ -77,7 +77,7 @@ int batch_grid(bhx_batch* b, int* grid, int* block) {
     bhx_context* ctx = b->ctx;
     *block = b->launch.block > 0 ? b->launch.block : 256;
     int per_cu = 0;
-    HIPCHK(trace_batch_occupancy(*block, b->n, &per_cu));
+    HIPCHK(trace_batch_occupancy(*block, b->n, trace_is_lean(b->trace), &per_cu));
     if (per_cu <= 0) return bhx_fail(ctx, BHX_E_HIP, "batched tracer cannot be resident");
     if (b->launch.waves_per_simd > 0)
         per_cu = std::min(per_cu, std::max(1, b->launch.waves_per_simd * 4 * 64 / *block));
@@ -279,6 +279,13 @@ int bhx_batch_create(bhx_context* ctx, const bhx_scene* scenes, int32_t n, bhx_b
     BCHK(hipMemcpy(b->table.p, table.data(), sizeof(SceneK) * n, hipMemcpyHostToDevice));
 
     TraceArgs& P = b->trace;
+    // the uniform event flags say whether any scene has jets or the Sun (which
+    // tracer kernel the batch needs); each lane reads its own scene's flags
+    P.has_jet = P.has_sun = 0;
+    for (const SceneK& k : table) {
+        P.has_jet |= k.has_jet;
+        P.has_sun |= k.has_sun;
+    }
     P.n_rays = static_cast<int32_t>(N);
     P.init = static_cast<const double*>(b->init.p);
     P.q2ray = static_cast<const int32_t*>(b->q2ray.p);

@@ -21,7 +21,7 @@ hipError_t trace_occupancy(int block, bool lean, int* blocks_per_cu);
 bool trace_is_lean(const TraceArgs& args);
 hipError_t launch_pilot(const RaygenArgs& gen, const TraceArgs& args, int block, hipStream_t stream);
 hipError_t launch_trace_batch(const TraceArgs& args, int grid, int block, hipStream_t stream);
-hipError_t trace_batch_occupancy(int block, int n_scenes, int* blocks_per_cu);
+hipError_t trace_batch_occupancy(int block, int n_scenes, bool lean, int* blocks_per_cu);
 hipError_t launch_shade(const ShadeArgs& args, int blocks, hipStream_t s);
 hipError_t launch_develop(const double* xyz, int64_t n, double exposure, int tm, double log_k,
                           uint8_t* rgb, hipStream_t s);

@@ -701,6 +701,8 @@ k_trace_pilot(const TraceArgs P) {
 }
 
 // Batched: the scene table (n_scenes rows) is copied into dynamic LDS first.
+// kLean: no scene of the batch has jets or the Sun.
+template <bool kLean>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 k_trace_batch(const TraceArgs P) {
     __shared__ unsigned long long blk[4];
@@ -713,7 +715,7 @@ k_trace_batch(const TraceArgs P) {
         for (int i = threadIdx.x; i < nd; i += blockDim.x) sk_raw[i] = src[i];
     }
     __syncthreads();
-    trace_body<true>(P, sk, sk_raw + P.n_scenes * (sizeof(SceneK) / sizeof(double)), blk);
+    trace_body<true, false, kLean>(P, sk, sk_raw + P.n_scenes * (sizeof(SceneK) / sizeof(double)), blk);
     flush_totals(P, blk);
 }
 
@@ -753,9 +755,13 @@ hipError_t launch_pilot(const RaygenArgs& gen, const TraceArgs& args, int block,
     return hipGetLastError();
 }
 
+// The batch's has_jet / has_sun say whether any of its scenes has them.
 hipError_t launch_trace_batch(const TraceArgs& args, int grid, int block, hipStream_t stream) {
-    const size_t lds = sizeof(SceneK) * (size_t)args.n_scenes + pend_bytes(block);
-    hipLaunchKernelGGL(k_trace_batch, dim3(grid), dim3(block), lds, stream, args);
+    const size_t table = sizeof(SceneK) * (size_t)args.n_scenes;
+    if (trace_is_lean(args))
+        hipLaunchKernelGGL(k_trace_batch<true>, dim3(grid), dim3(block), table, stream, args);
+    else
+        hipLaunchKernelGGL(k_trace_batch<false>, dim3(grid), dim3(block), table + pend_bytes(block), stream, args);
     return hipGetLastError();
 }
 
@@ -764,9 +770,11 @@ hipError_t trace_occupancy(int block, bool lean, int* blocks_per_cu) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_trace, block, pend_bytes(block));
 }
 
-hipError_t trace_batch_occupancy(int block, int n_scenes, int* blocks_per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_trace_batch, block,
-                                                        sizeof(SceneK) * (size_t)n_scenes + pend_bytes(block));
+hipError_t trace_batch_occupancy(int block, int n_scenes, bool lean, int* blocks_per_cu) {
+    const size_t table = sizeof(SceneK) * (size_t)n_scenes;
+    if (lean) return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_trace_batch<true>, block, table);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_trace_batch<false>, block,
+                                                        table + pend_bytes(block));
 }
 
 }  // namespace bhx
