@@ -29,10 +29,12 @@ ALG_FLOPS_PER_RHS = 151
 ALG_FLOPS_PER_ACCEPTED_STEP = 1659
 ALG_FLOPS_PER_REJECTED_ATTEMPT = 1417
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector peak (AMD published; FP64 matrix peak is the same)
-# HBM bytes per k_trace launch at this workload, from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE runs scaled by a copy kernel of known size (the file says how).
+# HBM bytes per tracer launch at this workload (the frame has no jets and no
+# Sun, so the library runs the tracer kernel without those branches,
+# k_trace_lean), from rocprofv3 FETCH_SIZE and WRITE_SIZE runs scaled by a copy
+# kernel of known size (the file says how).
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_trace_sky_frame_1024_traffic.txt")
-# Executed FP64 operations per k_trace launch at this workload, from the
+# Executed FP64 operations per k_trace_lean launch at this workload, from the
 # rocprofv3 counters SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 (FMA = 2) scaled
 # to lanes by the measured lane utilisation (the file says how).
 COUNTERS_FILE = os.path.join(ROOT, "profiles", "k_trace_sky_frame_1024_counters.txt")
@@ -56,7 +58,7 @@ def profile_values(path):
 
 
 def traffic_per_launch():
-    """HBM bytes per k_trace launch recorded in TRAFFIC_FILE, or None."""
+    """HBM bytes per tracer launch recorded in TRAFFIC_FILE, or None."""
     v = profile_values(TRAFFIC_FILE)
     if "fetch_bytes_per_launch" in v and "write_bytes_per_launch" in v:
         return v["fetch_bytes_per_launch"] + v["write_bytes_per_launch"]
@@ -64,7 +66,7 @@ def traffic_per_launch():
 
 
 def executed_flops_per_launch():
-    """Counter-measured FP64 operations per k_trace launch (COUNTERS_FILE), or None."""
+    """Counter-measured FP64 operations per tracer launch (COUNTERS_FILE), or None."""
     return profile_values(COUNTERS_FILE).get("executed_fp64_flops_per_launch")
 
 
@@ -535,9 +537,9 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     # device time of the whole trace call: counter fill, the cost-ordered
-    # queue's pilot + sort + scan, k_raygen and k_trace
+    # queue's pilot + sort + scan, k_raygen and the tracer kernel
     trace_call_ms = sum(a.elapsed_time(b) for a, b in ev) / args.steps
-    # k_trace alone (and the queue ordering alone), from the HIP events the
+    # the tracer kernel alone (and the queue ordering alone), from the HIP events the
     # library records on the launch stream around each kernel; synchronous
     # calls, so kept out of the timed loop above
     reps = max(1, min(args.steps, 20))
@@ -603,7 +605,7 @@ def main():
         "grad_over_fwd": None,
         "roofline": {"bound": "fp64-vector", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                     "traffic": traffic_per_launch(), "kernel": "k_trace",
+                     "traffic": traffic_per_launch(), "kernel": "k_trace_lean",
                      "pipe": "FP64 compute-bound on the vector ALUs: no MFMA (no dense "
                              "contraction on this path); 78.6 TFLOP/s is MI355X's FP64 vector "
                              "peak, equal to its FP64 matrix peak",
