@@ -29,10 +29,12 @@ HIP_SOURCES = ["tracer.hip", "shade.hip", "api.hip", "order.hip", "batch.hip", "
 # display.hip: the glare's sums must be the reference's IEEE sequence (no FMA).
 # shade.hip: the same LICM setting as the tracer's (the shading loop inlines the
 # library's acos / atan2 / exp / log / pow, whose constants would otherwise be
-# hoisted into registers: 256 VGPRs and scratch spills instead of 146 and none).
+# hoisted into registers: 256 VGPRs and scratch spills instead of 144 and none),
+# and per-expression FMAs like the tracer: the single-scene and the batched
+# shading kernel inline one body and must round alike.
 EXTRA = {"tracer.hip": ["-mllvm", "-disable-machine-licm", "-ffp-contract=on"],
          "noalloc.hip": ["-ffp-contract=off"],
-         "shade.hip": ["-mllvm", "-disable-machine-licm"],
+         "shade.hip": ["-mllvm", "-disable-machine-licm", "-ffp-contract=on"],
          "adjoint.hip": ["-mllvm", "-disable-machine-licm", "-ffp-contract=on"],
          "display.hip": ["-ffp-contract=off"]}
 CXX_SOURCES = ["host_setup.cpp", "host_physics.cpp"]

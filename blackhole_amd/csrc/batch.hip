@@ -50,6 +50,10 @@ struct bhx_batch {
     DevBuf tile_order, tile_start, init, q2ray, q2scene, table;
     DevBuf cls, y, jet, steps, attempts, norm, carter, xyz;
     DevBuf counters, flux, cell_start, star_rows;
+    // one launch of ray generation / of shading for every scene: the scenes'
+    // arguments and each block's scene and first block of that scene
+    DevBuf gen_table, gen_map, shd_table, shd_map;
+    int gen_blocks = 0;
     bhx_launch launch{};
     hipStream_t cap = nullptr;
     hipGraph_t graph = nullptr;
@@ -96,9 +100,13 @@ int enqueue(bhx_batch* b, hipStream_t st) {
     P.chunk = std::max(1, b->launch.chunk);
     HIPCHK(launch_zero_u64(static_cast<unsigned long long*>(b->counters.p),
                            static_cast<int>(counter_words(b->n)), st));
-    for (int i : b->queue_order) HIPCHK(launch_raygen(b->gen[i], st));
+    const int32_t* gmap = static_cast<const int32_t*>(b->gen_map.p);
+    HIPCHK(launch_raygen_batch(static_cast<const RaygenArgs*>(b->gen_table.p), gmap, gmap + b->gen_blocks,
+                               b->gen_blocks, st));
     HIPCHK(launch_trace_batch(P, grid, block, st));
-    for (int i = 0; i < b->n; ++i) HIPCHK(launch_shade(b->shd[i], b->shade_blocks[i], st));
+    const int32_t* smap = static_cast<const int32_t*>(b->shd_map.p);
+    HIPCHK(launch_shade_batch(static_cast<const ShadeArgs*>(b->shd_table.p), smap, smap + b->flux_total,
+                              b->flux_total, st));
     return BHX_OK;
 }
 
@@ -275,6 +283,30 @@ int bhx_batch_create(bhx_context* ctx, const bhx_scene* scenes, int32_t n, bhx_b
     b->flux_total = foff;
     BCHK(b->flux.ensure(sizeof(double) * foff));
     for (int i = 0; i < n; ++i) b->shd[i].flux_partial = static_cast<double*>(b->flux.p) + b->flux_off[i];
+    // the batched launches' tables: block -> scene maps followed by each
+    // scene's first block (ray generation in queue order, shading by scene)
+    {
+        std::vector<int32_t> gmap, gfirst(n), smap, sfirst(n);
+        for (int i : b->queue_order) {
+            gfirst[i] = static_cast<int32_t>(gmap.size());
+            gmap.insert(gmap.end(), ntiles[i], i);
+        }
+        for (int i = 0; i < n; ++i) {
+            sfirst[i] = b->flux_off[i];
+            smap.insert(smap.end(), b->shade_blocks[i], i);
+        }
+        b->gen_blocks = static_cast<int>(gmap.size());
+        gmap.insert(gmap.end(), gfirst.begin(), gfirst.end());
+        smap.insert(smap.end(), sfirst.begin(), sfirst.end());
+        BCHK(b->gen_table.ensure(sizeof(RaygenArgs) * n));
+        BCHK(b->gen_map.ensure(sizeof(int32_t) * gmap.size()));
+        BCHK(b->shd_table.ensure(sizeof(ShadeArgs) * n));
+        BCHK(b->shd_map.ensure(sizeof(int32_t) * smap.size()));
+        BCHK(hipMemcpy(b->gen_table.p, b->gen.data(), sizeof(RaygenArgs) * n, hipMemcpyHostToDevice));
+        BCHK(hipMemcpy(b->gen_map.p, gmap.data(), sizeof(int32_t) * gmap.size(), hipMemcpyHostToDevice));
+        BCHK(hipMemcpy(b->shd_table.p, b->shd.data(), sizeof(ShadeArgs) * n, hipMemcpyHostToDevice));
+        BCHK(hipMemcpy(b->shd_map.p, smap.data(), sizeof(int32_t) * smap.size(), hipMemcpyHostToDevice));
+    }
     BCHK(b->table.ensure(sizeof(SceneK) * n));
     BCHK(hipMemcpy(b->table.p, table.data(), sizeof(SceneK) * n, hipMemcpyHostToDevice));
 
@@ -316,7 +348,8 @@ int bhx_batch_destroy(bhx_batch* b) {
     drop_graph(b);
     for (DevBuf* d : {&b->tile_order, &b->tile_start, &b->init, &b->q2ray, &b->q2scene, &b->table,
                       &b->cls, &b->y, &b->jet, &b->steps, &b->attempts, &b->norm, &b->carter, &b->xyz,
-                      &b->counters, &b->flux, &b->cell_start, &b->star_rows})
+                      &b->counters, &b->flux, &b->cell_start, &b->star_rows, &b->gen_table, &b->gen_map,
+                      &b->shd_table, &b->shd_map})
         d->release();
     if (b->cap) (void)hipStreamDestroy(b->cap);
     for (auto& e : b->ev)

@@ -14,6 +14,12 @@
 
 namespace bhx {
 hipError_t launch_raygen(const RaygenArgs& args, hipStream_t stream);
+// Ray generation / shading of many scenes in one launch: block b works for
+// scene block_scene[b], as that scene's block b - block_first[scene].
+hipError_t launch_raygen_batch(const RaygenArgs* table, const int32_t* block_scene, const int32_t* block_first,
+                               int blocks, hipStream_t stream);
+hipError_t launch_shade_batch(const ShadeArgs* table, const int32_t* block_scene, const int32_t* block_first,
+                              int blocks, hipStream_t s);
 hipError_t launch_trace(const TraceArgs& args, int grid, int block, hipStream_t stream);
 // Tracer blocks resident per CU, for the lean kernel (scenes without jets or
 // the Sun, three waves per SIMD) or the full one (two).

@@ -132,14 +132,15 @@ __device__ void sky_radiance(const ShadeArgs& P, const double d[3], double o[3])
 // constants of every inlined acos / atan2 / exp / log / pow out of the
 // per-sample loop into registers for the whole kernel, which took 256 VGPRs
 // and 180 bytes of scratch per lane at two waves per SIMD.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_shade(const ShadeArgs P) {
+// Block `blk` of a scene's shading (256 pixels).
+__device__ __forceinline__ void shade_block(const ShadeArgs& P, int blk) {
     __shared__ unsigned long long counts[7];
     __shared__ double flux_w[4];
     __shared__ unsigned long long ttot[4];
     if (threadIdx.x < 7) counts[threadIdx.x] = 0;
     if (threadIdx.x < 4) ttot[threadIdx.x] = 0;
     __syncthreads();
-    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int pix = blk * blockDim.x + threadIdx.x;
     const int S2 = P.S * P.S;
     const int64_t N = P.n_rays;
     unsigned int my_counts[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -335,8 +336,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     if (threadIdx.x == 0) {
         double s = 0.0;
         for (int w = 0; w < (int)(blockDim.x / 64); ++w) s += flux_w[w];
-        P.flux_partial[blockIdx.x] = s;
+        P.flux_partial[blk] = s;
     }
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_shade(const ShadeArgs P) {
+    shade_block(P, blockIdx.x);
+}
+
+// Every scene of a batch in one launch: block b shades block b - first[s] of
+// scene s = scene[b] (scene arguments from the table).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+k_shade_batch(const ShadeArgs* __restrict__ table, const int32_t* __restrict__ scene,
+              const int32_t* __restrict__ first) {
+    const int s = scene[blockIdx.x];
+    shade_block(table[s], (int)blockIdx.x - first[s]);
 }
 
 // ---- display ---------------------------------------------------------------
@@ -509,6 +523,11 @@ __global__ void k_fill_u32(unsigned int* p, int n) {
 
 hipError_t launch_shade(const ShadeArgs& args, int blocks, hipStream_t s) {
     hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(256), 0, s, args);
+    return hipGetLastError();
+}
+hipError_t launch_shade_batch(const ShadeArgs* table, const int32_t* block_scene, const int32_t* block_first,
+                              int blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_shade_batch, dim3(blocks), dim3(256), 0, s, table, block_scene, block_first);
     return hipGetLastError();
 }
 hipError_t launch_develop(const double* xyz, int64_t n, double exposure, int tm, double log_k,

@@ -87,8 +87,8 @@ __device__ __forceinline__ void gen_sample(const RaygenArgs& P, int px, int py, 
     P.q2ray[q] = ray;
 }
 
-__global__ void __launch_bounds__(64) k_raygen(const RaygenArgs P) {
-    const int t = blockIdx.x;
+// Queue tile t of a scene: its 8 x 8 pixels' samples at their queue slots.
+__device__ __forceinline__ void raygen_tile(const RaygenArgs& P, int t) {
     if (t >= P.n_tiles) return;
     const int tile = P.tile_order[t];
     const int tx = tile % P.tiles_x, ty = tile / P.tiles_x;
@@ -105,6 +105,17 @@ __global__ void __launch_bounds__(64) k_raygen(const RaygenArgs P) {
         gen_sample(P, px, py, k, q, P.ray_base + (py * P.W + px) * S2 + k, N);
         if (P.q2scene) P.q2scene[q] = (uint8_t)P.scene_id;
     }
+}
+
+__global__ void __launch_bounds__(64) k_raygen(const RaygenArgs P) { raygen_tile(P, blockIdx.x); }
+
+// Every scene of a batch in one launch: block b generates queue tile
+// b - first[s] of scene s = scene[b] (scene arguments from the table).
+__global__ void __launch_bounds__(64) k_raygen_batch(const RaygenArgs* __restrict__ table,
+                                                     const int32_t* __restrict__ scene,
+                                                     const int32_t* __restrict__ first) {
+    const int s = scene[blockIdx.x];
+    raygen_tile(table[s], (int)blockIdx.x - first[s]);
 }
 
 // Pilot rays of the cost-ordered queue: one per 8 x 8 tile, at the tile's
@@ -153,7 +164,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
     // slot my_pend; a ray that ended writes its jet integral at the next loop
     // head (jet_out), after that sample
     bool jet_pend = false, jet_out = false;
-    double* my_pend = pend + threadIdx.x * kPend;
+    double* my_pend = kLean ? nullptr : pend + threadIdx.x * kPend;
 #pragma unroll
     for (int c = 0; c < 6; ++c) { y[c] = 0.0; k1[c] = 0.0; }
 
@@ -723,6 +734,12 @@ k_trace_batch(const TraceArgs P) {
 
 hipError_t launch_raygen(const RaygenArgs& args, hipStream_t stream) {
     hipLaunchKernelGGL(k_raygen, dim3(args.n_tiles), dim3(64), 0, stream, args);
+    return hipGetLastError();
+}
+
+hipError_t launch_raygen_batch(const RaygenArgs* table, const int32_t* block_scene, const int32_t* block_first,
+                               int blocks, hipStream_t stream) {
+    hipLaunchKernelGGL(k_raygen_batch, dim3(blocks), dim3(64), 0, stream, table, block_scene, block_first);
     return hipGetLastError();
 }
 
