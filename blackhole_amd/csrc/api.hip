@@ -268,6 +268,13 @@ int bhx_trace_dev(bhx_context* ctx, const bhx_scene* s, const bhx_rays* rays, bh
     int grid = 0, block = 0;
     rc = trace_grid(ctx, trace_is_lean(P), &grid, &block);
     if (rc) return rc;
+    // a frame with fewer rays than resident lanes: one-wave blocks, as many as
+    // its rays fill, so every wave holds 64 rays and runs without a half-empty
+    // partner sharing its SIMD's issue slots (the 256^2 frame: 2.37 -> 2.03 ms)
+    if (n_rays < static_cast<int64_t>(grid) * block) {
+        block = 64;
+        grid = static_cast<int>((n_rays + 63) / 64);
+    }
     // counters are cleared by a fill kernel (never a memset node: the call may be captured)
     HIPCHK(launch_zero_u64(reinterpret_cast<unsigned long long*>(ctx->d_cnt), sizeof(Counters) / 8, st));
     // longest-first queue when every resident lane sees several rays (below

@@ -215,7 +215,13 @@ extern "C" int bhx_trace_noalloc_dev(int32_t device, const bhx_scene* s, const b
     if (launch_zero_u64(reinterpret_cast<unsigned long long*>(cnt), sizeof(Counters) / 8, st) != hipSuccess)
         return fail(BHX_E_HIP, "counter clear failed");
     if (launch_raygen(G, st) != hipSuccess) return fail(BHX_E_HIP, "ray generation launch failed");
-    if (launch_trace(P, trace_is_lean(P) ? d.grid_lean : d.grid, kBlock, st) != hipSuccess)
+    // fewer rays than resident lanes: one-wave blocks, as bhx_trace_dev does
+    int grid = trace_is_lean(P) ? d.grid_lean : d.grid, block = kBlock;
+    if (n_rays < static_cast<int64_t>(grid) * block) {
+        block = 64;
+        grid = (n_rays + 63) / 64;
+    }
+    if (launch_trace(P, grid, block, st) != hipSuccess)
         return fail(BHX_E_HIP, "tracer launch failed");
     if (stats) {
         Counters c;
