@@ -375,7 +375,11 @@ __host__ __device__ inline void rhs_vjp(const RayK& K, const double x[4], const 
     const double rpa = r2 + a2;
     const double S = r2 + a2 * c2;
     const double D = rpa - 2.0 * r;
-    const double iS = 1.0 / S, iD = 1.0 / D, is2 = 1.0 / s2;
+    // one reciprocal of Sigma Delta sin^2 for the three (as kerr_rhs_sc forms
+    // them; all three factors are positive on a ray outside the horizon)
+    const double SD = S * D;
+    const double iX = rcp(SD * s2);
+    const double iS = D * s2 * iX, iD = S * s2 * iX, is2 = SD * iX;
     const double P = rpa * pt + a * pph;
     const double PiD = P * iD;
     const double U = pph + a * pt * s2;
