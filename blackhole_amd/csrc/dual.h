@@ -1,14 +1,18 @@
-// dual.h - forward-mode dual numbers for the once-per-ray derivative pieces
-// of the adjoint (camera frame, disc emission).  The per-step derivatives
-// (Hamiltonian right-hand side, Dormand-Prince stages, dense output) are
-// written out in reverse mode by hand in adjoint.hip; these cover the short
-// closed-form chains at the ends of a ray, where N tangent directions cost
-// less than a hand derivation is worth.
+// dual.h - forward-mode dual numbers for the closed-form derivative pieces of
+// the adjoint (camera frame, disc emission, the conical jet's integrand at
+// each quadrature node).  The per-step derivatives (Hamiltonian right-hand
+// side, Dormand-Prince stages, dense output) are written out in reverse mode
+// by hand in adjoint.hip; these cover short closed-form chains, where N
+// tangent directions cost less than a hand derivation is worth.  A quotient
+// takes one reciprocal (hardware estimate + Newton, <= 1 ulp) for its value
+// and its derivative instead of the IEEE division sequence per term.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+
+#include "kerr_dev.h"
 
 namespace bhx {
 
@@ -69,8 +73,9 @@ __host__ __device__ inline Dual<N> operator*(const Dual<N>& a, const Dual<N>& b)
 }
 template <int N>
 __host__ __device__ inline Dual<N> operator/(const Dual<N>& a, const Dual<N>& b) {
-    const double q = a.v / b.v;
-    return scale_add(a, 1.0 / b.v, b, -q / b.v, q);
+    const double ib = rcp(b.v);
+    const double q = a.v * ib;
+    return scale_add(a, ib, b, -q * ib, q);
 }
 template <int N>
 __host__ __device__ inline Dual<N> operator-(const Dual<N>& a) {
@@ -89,20 +94,24 @@ __host__ __device__ inline Dual<N> operator*(const Dual<N>& a, double b) { retur
 template <int N>
 __host__ __device__ inline Dual<N> operator*(double b, const Dual<N>& a) { return chain(a, a.v * b, b); }
 template <int N>
-__host__ __device__ inline Dual<N> operator/(const Dual<N>& a, double b) { return chain(a, a.v / b, 1.0 / b); }
+__host__ __device__ inline Dual<N> operator/(const Dual<N>& a, double b) {
+    const double ib = rcp(b);
+    return chain(a, a.v * ib, ib);
+}
 template <int N>
 __host__ __device__ inline Dual<N> operator/(double b, const Dual<N>& a) {
-    const double q = b / a.v;
-    return chain(a, q, -q / a.v);
+    const double ia = rcp(a.v);
+    const double q = b * ia;
+    return chain(a, q, -q * ia);
 }
 
 template <int N>
 __host__ __device__ inline Dual<N> sqrt(const Dual<N>& a) {
     const double s = ::sqrt(a.v);
-    return chain(a, s, 0.5 / s);
+    return chain(a, s, 0.5 * rcp(s));
 }
 template <int N>
-__host__ __device__ inline Dual<N> log(const Dual<N>& a) { return chain(a, ::log(a.v), 1.0 / a.v); }
+__host__ __device__ inline Dual<N> log(const Dual<N>& a) { return chain(a, ::log(a.v), rcp(a.v)); }
 template <int N>
 __host__ __device__ inline Dual<N> exp(const Dual<N>& a) {
     const double e = ::exp(a.v);
@@ -119,12 +128,12 @@ __host__ __device__ inline Dual<N> acos(const Dual<N>& a) {
 template <int N>
 __host__ __device__ inline Dual<N> cbrt(const Dual<N>& a) {
     const double c = ::cbrt(a.v);
-    return chain(a, c, 1.0 / (3.0 * c * c));
+    return chain(a, c, rcp(3.0 * c * c));
 }
 template <int N>
 __host__ __device__ inline Dual<N> pow(const Dual<N>& a, double p) {
     const double y = ::pow(a.v, p);
-    return chain(a, y, p * y / a.v);
+    return chain(a, y, p * y * rcp(a.v));
 }
 
 __host__ __device__ inline double value(double x) { return x; }
