@@ -54,9 +54,9 @@ constexpr int kMaxBatchScenes = 128;
 // cap kPilotHmax times the reference's hmax (render.cpp:130-131), at least
 // these tolerances, at most kPilotSteps accepted steps.  Its attempt counts
 // only rank the tiles.
-constexpr double kPilotHmax = 16.0;
+constexpr double kPilotHmax = 24.0;
 constexpr double kPilotRtol = 1e-4, kPilotAtol = 1e-6;
-constexpr int kPilotSteps = 32;
+constexpr int kPilotSteps = 24;
 
 // Ray generation: camera, jitter and the initial photon of every sample
 // (src/render.cpp:437-452, src/scene.h:356-384, src/geodesic.h:215-231),
