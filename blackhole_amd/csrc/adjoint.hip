@@ -14,12 +14,15 @@
 //                   8-record chunks that each ray takes as it goes (an atomic
 //                   counter), each chunk linked back to the ray's previous
 //                   one: the pool holds what the rays' real step counts need.
-//   k_adj_backward  one lit ray (I > 0) per lane, sorted by step count so the
-//                   lanes of a wave replay and sweep segments of equal length
-//                   together (a wave's few jet-lit lanes share their jet
-//                   samples' node evaluations with the other lanes); each lane's (K + 1)-step segment buffer is a strided
-//                   HBM scratch.  Per-block partial gradients, summed on the
-//                   host in a fixed order.
+//   k_adj_backward  one lit ray (I > 0) per lane, the rays sorted by step
+//                   count and taken 64 at a time by whichever wave is done,
+//                   so the lanes of a wave replay and sweep segments of equal
+//                   length together and the waves holding the longest rays
+//                   take nothing else (a wave's few jet-lit lanes share their
+//                   jet samples' node evaluations with the other lanes); each
+//                   lane's (K + 1)-step segment buffer is a strided HBM
+//                   scratch.  Each ray's contribution is kept and summed per
+//                   frame in a fixed order (k_adj_frame_sums).
 #include "adjoint_frame.h"
 #include "context.h"
 
@@ -121,12 +124,12 @@ struct AdjArgs {
     const int32_t* order;     // rays by key, descending: the lit rays first
     double* seg;              // (K + 1) * kSeg doubles per lane, strided by seg_stride
     int64_t seg_stride;
-    double* partial;          // 4 per block (one frame)
+    unsigned long long* q_next;   // the backward's next slot of order (per wave, 64 at a time)
     unsigned int* mismatch;
     // frames of a batch (kernels instantiated for batches): ray r belongs to
     // frame r / n_per, whose camera is cams[r / n_per]; the backward leaves
     // each processed ray's d(w I)/d(params) in contrib (4 x n_rays SoA) for
-    // the per-frame sums
+    // the per-frame sums (one frame: n_per = n_rays)
     const FrameCam* cams;
     int64_t n_per;
     double* contrib;
@@ -271,50 +274,46 @@ __global__ void k_iota(int32_t* p, int64_t n) {
 
 template <bool kBatch>
 __global__ void __launch_bounds__(256) k_adj_backward(const AdjConst* __restrict__ C, const AdjArgs P) {
-    __shared__ double part[4][4];
     __shared__ double jet_lds[4][kJetLds];   // per wave: the jet nodes shared across its lanes
     const Frame& F = C->F;
     const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t nl = (int64_t)gridDim.x * blockDim.x;
-    double g[4] = {0.0, 0.0, 0.0, 0.0};
     unsigned int mism = 0;
     const int64_t n_order = *P.n_lit;
-    for (int64_t i = lane; i < n_order; i += nl) {
+    // A wave takes the next 64 rays of the order (most accepted steps first)
+    // whenever all its lanes are done: the waves holding the longest rays
+    // take nothing more, the others share out the rest.  Every processed ray
+    // leaves its own contribution, summed per frame afterwards in a fixed
+    // order, so the gradient does not depend on which lane took which ray.
+    for (;;) {
+        unsigned long long base = 0;
+        if (__lane_id() == 0) base = atomicAdd(P.q_next, 64ull);
+        base = __shfl(base, 0);
+        if (base >= static_cast<unsigned long long>(n_order)) break;
+        const int64_t i = static_cast<int64_t>(base) + __lane_id();
+        if (i >= n_order) continue;
         const int64_t ray = P.order[i];
         const double w = P.w[ray];
         if (w == 0.0) continue;
         const PoolReader rd{P.pool, P.link, P.ck_last[ray], P.ck_nrec[ray], P.ck_lg};
+        double gr[4] = {0.0, 0.0, 0.0, 0.0};
         if (kBatch) {
-            // this ray's own contribution, summed per frame afterwards
             const int64_t frame = ray / P.n_per;
-            double gr[4] = {0.0, 0.0, 0.0, 0.0};
             mism += bwd_ray(F, P.cams[frame], ray - frame * P.n_per, P.steps[ray], P.cls[ray], w, rd, P.seg + lane,
                             P.seg_stride, gr, F.has_jet ? jet_lds[threadIdx.x / 64] : nullptr);
-            for (int c = 0; c < 4; ++c) P.contrib[c * P.n_rays + ray] = gr[c];
         } else {
-            mism += bwd_ray(F, C->cam, ray, P.steps[ray], P.cls[ray], w, rd, P.seg + lane, P.seg_stride, g,
+            mism += bwd_ray(F, C->cam, ray, P.steps[ray], P.cls[ray], w, rd, P.seg + lane, P.seg_stride, gr,
                             F.has_jet ? jet_lds[threadIdx.x / 64] : nullptr);
         }
+        for (int c = 0; c < 4; ++c) P.contrib[c * P.n_rays + ray] = gr[c];
     }
-    // block sums in a fixed order
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-        for (int off = 32; off > 0; off >>= 1) g[c] += __shfl_xor(g[c], off);
-    if (__lane_id() == 0)
-        for (int c = 0; c < 4; ++c) part[threadIdx.x / 64][c] = g[c];
     for (int off = 32; off > 0; off >>= 1) mism += __shfl_xor(mism, off);
     if (__lane_id() == 0 && mism) atomicAdd(P.mismatch, mism);
-    __syncthreads();
-    if (threadIdx.x < 4) {
-        double sum = 0.0;
-        for (int wv = 0; wv < (int)(blockDim.x / 64); ++wv) sum += part[wv][threadIdx.x];
-        P.partial[blockIdx.x * 4 + threadIdx.x] = sum;
-    }
 }
 
-// Per frame of a batch, the sum of its processed rays' contributions (lit
-// rays with dL/dI != 0), in a fixed order: one block per frame, each thread a
-// fixed stride of the frame's rays, then a fixed tree; deterministic.
+// Per frame (of a batch, or the one frame), the sum of its processed rays'
+// contributions (lit rays with dL/dI != 0), in a fixed order: one block per
+// frame, each thread a fixed stride of the frame's rays, then a fixed tree;
+// deterministic.
 __global__ void __launch_bounds__(256) k_adj_frame_sums(const AdjArgs P, const double* __restrict__ w,
                                                         double* __restrict__ out) {
     __shared__ double red[4][256];
@@ -347,7 +346,7 @@ struct bhx_adjoint {
     int64_t n_rays = 0;              // samples of all frames
     AdjConst host{};
     DevBuf cst, cams, pool, link, ck_last, ck_nrec, cls, y, steps, I, key, key_sorted, iota, order, sort_tmp, seg,
-        partial, counters, qmap, contrib, frame_grad;
+        counters, qmap, contrib, frame_grad;
     uint32_t pool_cap = 0;           // chunks in the checkpoint pool (>= n_rays: a first chunk per ray)
     int32_t q_chunk = 32;            // the forward's queue: slots per wave take
     int32_t refill = 1;              // idle lanes a wave waits for before it refills them
@@ -491,7 +490,7 @@ int adj_sweep(bhx_adjoint* A, const double* w, double* grad, hipStream_t st) {
     P.order = static_cast<const int32_t*>(A->order.p);
     P.seg = static_cast<double*>(A->seg.p);
     P.seg_stride = static_cast<int64_t>(A->grid_b) * 256;
-    P.partial = static_cast<double*>(A->partial.p);
+    P.q_next = reinterpret_cast<unsigned long long*>(static_cast<char*>(A->counters.p) + 40);
     P.mismatch = reinterpret_cast<unsigned int*>(static_cast<char*>(A->counters.p) + 32);
     P.n_lit = reinterpret_cast<unsigned int*>(static_cast<char*>(A->counters.p) + 8);
     P.cams = static_cast<const FrameCam*>(A->cams.p);
@@ -499,33 +498,23 @@ int adj_sweep(bhx_adjoint* A, const double* w, double* grad, hipStream_t st) {
     P.contrib = static_cast<double*>(A->contrib.p);
     const int n_frames = static_cast<int>(A->scenes.size());
     HIPCHK(launch_zero_u32(P.mismatch, 1, st));
+    HIPCHK(launch_zero_u64(P.q_next, 1, st));
     HIPCHK(hipEventRecord(A->ev[2], st));
-    if (A->is_batch) {
+    if (A->is_batch)
         hipLaunchKernelGGL(k_adj_backward<true>, dim3(A->grid_b), dim3(256), 0, st,
                            static_cast<const AdjConst*>(A->cst.p), P);
-        HIPCHK(hipGetLastError());
-        hipLaunchKernelGGL(k_adj_frame_sums, dim3(n_frames), dim3(256), 0, st, P, w,
-                           static_cast<double*>(A->frame_grad.p));
-    } else {
+    else
         hipLaunchKernelGGL(k_adj_backward<false>, dim3(A->grid_b), dim3(256), 0, st,
                            static_cast<const AdjConst*>(A->cst.p), P);
-    }
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_adj_frame_sums, dim3(n_frames), dim3(256), 0, st, P, w,
+                       static_cast<double*>(A->frame_grad.p));
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(A->ev[3], st));
     unsigned int mism = 0;
-    if (A->is_batch) {
-        HIPCHK(hipMemcpyAsync(grad, A->frame_grad.p, sizeof(double) * 4 * n_frames, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(&mism, P.mismatch, sizeof mism, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-    } else {
-        std::vector<double> part(4 * static_cast<size_t>(A->grid_b));
-        HIPCHK(hipMemcpyAsync(part.data(), A->partial.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(&mism, P.mismatch, sizeof mism, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        for (int c = 0; c < 4; ++c) grad[c] = 0.0;
-        for (int b = 0; b < A->grid_b; ++b)
-            for (int c = 0; c < 4; ++c) grad[c] += part[4 * b + c];
-    }
+    HIPCHK(hipMemcpyAsync(grad, A->frame_grad.p, sizeof(double) * 4 * n_frames, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&mism, P.mismatch, sizeof mism, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     A->mismatches = mism;
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, A->ev[2], A->ev[3]) == hipSuccess) A->bwd_ms = ms;
@@ -615,9 +604,8 @@ int adj_create(bhx_context* ctx, const bhx_scene* s, int32_t n, int32_t K, bool 
         A->key_sorted.ensure(sizeof(int32_t) * n_rays) != hipSuccess ||
         A->iota.ensure(sizeof(int32_t) * n_rays) != hipSuccess ||
         A->seg.ensure(sizeof(double) * kSeg * (K + 1) * lanes) != hipSuccess ||
-        A->partial.ensure(sizeof(double) * 4 * A->grid_b) != hipSuccess ||
-        (batch && A->contrib.ensure(sizeof(double) * 4 * n_rays) != hipSuccess) ||
-        (batch && A->frame_grad.ensure(sizeof(double) * 4 * n) != hipSuccess) || A->counters.ensure(64) != hipSuccess)
+        A->contrib.ensure(sizeof(double) * 4 * n_rays) != hipSuccess ||
+        A->frame_grad.ensure(sizeof(double) * 4 * n) != hipSuccess || A->counters.ensure(64) != hipSuccess)
         return bail(bhx_fail(ctx, BHX_E_NOMEM, "adjoint buffers"));
     for (auto& e : A->ev)
         if (hipEventCreate(&e) != hipSuccess) return bail(bhx_fail(ctx, BHX_E_HIP, "hipEventCreate"));
@@ -683,7 +671,7 @@ int bhx_adjoint_destroy(bhx_adjoint* A) {
     (void)hipSetDevice(A->ctx->device);
     (void)hipDeviceSynchronize();
     for (DevBuf* d : {&A->cst, &A->cams, &A->pool, &A->link, &A->ck_last, &A->ck_nrec, &A->cls, &A->y, &A->steps,
-                      &A->I, &A->key, &A->key_sorted, &A->iota, &A->order, &A->sort_tmp, &A->seg, &A->partial,
+                      &A->I, &A->key, &A->key_sorted, &A->iota, &A->order, &A->sort_tmp, &A->seg,
                       &A->counters, &A->qmap, &A->contrib, &A->frame_grad})
         d->release();
     for (auto& e : A->ev)
