@@ -232,7 +232,7 @@ def batched_leg(tr, torch, rounds=3):
     }
 
 
-def adjoint_leg(tr, torch, width=512, Ks=(16, 32, 64, 128, 256), rounds=3, fd=True, threads=16):
+def adjoint_leg(tr, torch, width=512, Ks=(1, 16, 32, 64, 128, 256), rounds=3, fd=True, threads=16):
     """Differentiable disc image at width^2 (a=0.7, i=45 deg, 0.1 Edd, r=500M;
     target a=0.9, i=50 deg): loss = sum (G*I - G*I_target)^2 with a Gaussian
     sigma = 2 px.  Times a forward render (the plain path: trace + shade); the

@@ -39,12 +39,12 @@ using namespace bhx::adj;
 
 namespace {
 
-// Checkpoint pool: chunks of 2^lg records of kCk doubles (lg from K: a ray
-// of typical length fills about one chunk); link[c] is the chunk its ray
-// filled before c (-1 for the ray's first).  Per ray, the last chunk and the
-// number of records.
+// Checkpoint pool: chunks of 2^lg records of kCk doubles (kStep with K = 1;
+// lg from K: a ray of typical length fills about one chunk); link[c] is the
+// chunk its ray filled before c (-1 for the ray's first).  Per ray, the last
+// chunk and the number of records.
 constexpr int64_t kMaxPoolBytes = int64_t(128) << 30;   // 128 GiB
-inline int chunk_lg(int K) { return K >= 64 ? 2 : (K >= 16 ? 3 : 4); }
+inline int chunk_lg(int K) { return K >= 64 ? 2 : (K >= 16 ? 3 : (K > 1 ? 4 : 7)); }
 
 // The forward's store of one ray: record j goes to slot j mod 2^lg of the
 // ray's current chunk.  A ray's first chunk is chunk q, q its queue slot
@@ -63,6 +63,7 @@ struct PoolWriter {
     uint32_t cap;              // chunks in the pool (>= n_first)
     uint32_t n_first;          // first chunks: one per queue slot
     int32_t lg;
+    int32_t rs;                // doubles per record
     int4* slot;
     __device__ void start(int32_t ray, int32_t q) { *slot = make_int4(q, 0, ray, 0); }
     __device__ double* put(int j) {
@@ -81,7 +82,7 @@ struct PoolWriter {
         }
         slot->x = cur;
         slot->y = j + 1;
-        return cur == kLost ? nullptr : pool + ((static_cast<int64_t>(cur) << lg) + (j & mask)) * kCk;
+        return cur == kLost ? nullptr : pool + ((static_cast<int64_t>(cur) << lg) + (j & mask)) * rs;
     }
 };
 
@@ -90,11 +91,20 @@ struct PoolWriter {
 struct PoolReader {
     const double* pool;
     const int32_t* link;
-    int32_t last, nrec, lg;
+    int32_t last, nrec, lg, rs;
     __device__ const double* get(int j) const {
         int32_t c = last;
         for (int k = (nrec - 1) >> lg; k > (j >> lg); --k) c = link[c];
-        return pool + ((static_cast<int64_t>(c) << lg) + (j & ((1 << lg) - 1))) * kCk;
+        return pool + ((static_cast<int64_t>(c) << lg) + (j & ((1 << lg) - 1))) * rs;
+    }
+    // the walk back from the last record (n = nrec): one link per chunk
+    __device__ CkPos rpos(int n) const { return {last, n - 1}; }
+    __device__ const double* at(const CkPos& p) const {
+        return pool + ((static_cast<int64_t>(p.c) << lg) + (p.j & ((1 << lg) - 1))) * rs;
+    }
+    __device__ void step_back(CkPos& p) const {
+        if ((p.j & ((1 << lg) - 1)) == 0) p.c = link[p.c];
+        --p.j;
     }
 };
 
@@ -103,7 +113,7 @@ struct AdjArgs {
     // forward
     uint32_t* counter;
     const int32_t* qmap;      // queue slot -> ray index
-    double* pool;             // checkpoint chunks: pool_cap x 2^ck_lg x kCk doubles
+    double* pool;             // checkpoint chunks: pool_cap x 2^ck_lg x ck_rs doubles
     int32_t* link;            // pool_cap chunk links
     uint32_t* pool_counter;   // chunks taken beyond the first ones (the pool is short when
                               // n_rays + this exceeds pool_cap)
@@ -111,6 +121,7 @@ struct AdjArgs {
     int32_t q_chunk;          // queue slots a wave takes at once
     int32_t refill;           // idle lanes a wave waits for before it refills them
     int32_t ck_lg;
+    int32_t ck_rs;            // doubles per record: kCk, or kStep with K = 1
     int32_t* ck_last;         // per ray: its last chunk (-1: records lost)
     int32_t* ck_nrec;         // per ray: records written
     uint8_t* cls;
@@ -163,7 +174,7 @@ k_adj_forward(const AdjConst* __restrict__ C, const AdjArgs P) {
     // intensity of a ray that ended, until its last jet sample is in
     __shared__ int4 lane_ray[256];
     __shared__ double lane_idisc[256];
-    PoolWriter wr{P.pool, P.link, P.pool_counter, P.pool_cap, static_cast<uint32_t>(P.n_rays), P.ck_lg,
+    PoolWriter wr{P.pool, P.link, P.pool_counter, P.pool_cap, static_cast<uint32_t>(P.n_rays), P.ck_lg, P.ck_rs,
                   &lane_ray[threadIdx.x]};
     // the wave's share of the queue: slots [c_next, c_end), taken q_chunk at
     // a time; idle lanes get consecutive slots (ballot + prefix rank), once
@@ -294,7 +305,7 @@ __global__ void __launch_bounds__(256) k_adj_backward(const AdjConst* __restrict
         const int64_t ray = P.order[i];
         const double w = P.w[ray];
         if (w == 0.0) continue;
-        const PoolReader rd{P.pool, P.link, P.ck_last[ray], P.ck_nrec[ray], P.ck_lg};
+        const PoolReader rd{P.pool, P.link, P.ck_last[ray], P.ck_nrec[ray], P.ck_lg, P.ck_rs};
         double gr[4] = {0.0, 0.0, 0.0, 0.0};
         if (kBatch) {
             const int64_t frame = ray / P.n_per;
@@ -342,6 +353,7 @@ struct bhx_adjoint {
     bool is_batch = false;
     int K = 0;
     int ck_lg = 3;                   // checkpoint records per pool chunk: 2^ck_lg
+    int ck_rs = kCk;                 // doubles per record (kStep with K = 1)
     int64_t n_per = 0;               // samples per frame
     int64_t n_rays = 0;              // samples of all frames
     AdjConst host{};
@@ -356,7 +368,7 @@ struct bhx_adjoint {
     bool have_steps = false;         // steps holds a forward's step counts (the next queue order)
     unsigned int mismatches = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    size_t chunk_bytes() const { return sizeof(double) * kCk * (size_t(1) << ck_lg); }
+    size_t chunk_bytes() const { return sizeof(double) * ck_rs * (size_t(1) << ck_lg); }
 };
 
 namespace {
@@ -409,6 +421,7 @@ int adj_record(bhx_adjoint* A, double* I, hipStream_t st) {
     P.q_chunk = A->q_chunk;
     P.refill = A->refill;
     P.ck_lg = A->ck_lg;
+    P.ck_rs = A->ck_rs;
     P.ck_last = static_cast<int32_t*>(A->ck_last.p);
     P.ck_nrec = static_cast<int32_t*>(A->ck_nrec.p);
     P.cls = static_cast<uint8_t*>(A->cls.p);
@@ -481,6 +494,7 @@ int adj_sweep(bhx_adjoint* A, const double* w, double* grad, hipStream_t st) {
     P.pool = static_cast<double*>(A->pool.p);
     P.link = static_cast<int32_t*>(A->link.p);
     P.ck_lg = A->ck_lg;
+    P.ck_rs = A->ck_rs;
     P.ck_last = static_cast<int32_t*>(A->ck_last.p);
     P.ck_nrec = static_cast<int32_t*>(A->ck_nrec.p);
     P.cls = static_cast<uint8_t*>(A->cls.p);
@@ -532,6 +546,7 @@ int adj_set_K(bhx_adjoint* A, int K) {
     HIPCHK(A->seg.ensure(sizeof(double) * kSeg * (K + 1) * lanes));
     A->K = K;
     A->ck_lg = chunk_lg(K);
+    A->ck_rs = K == 1 ? kStep : kCk;
     // at least a first chunk per ray
     if (static_cast<double>(A->n_rays) * A->chunk_bytes() > static_cast<double>(kMaxPoolBytes))
         return bhx_fail(ctx, BHX_E_TOO_LARGE, "checkpoint pool above 128 GiB: raise K");
@@ -567,6 +582,7 @@ int adj_create(bhx_context* ctx, const bhx_scene* s, int32_t n, int32_t K, bool 
     A->refill = batch ? 4 : 1;
     A->K = K;
     A->ck_lg = chunk_lg(K);
+    A->ck_rs = K == 1 ? kStep : kCk;
     A->n_per = n_per;
     A->n_rays = n_rays;
     auto bail = [&](int rc) {
@@ -583,10 +599,10 @@ int adj_create(bhx_context* ctx, const bhx_scene* s, int32_t n, int32_t K, bool 
     A->grid_b = per_cu * ctx->n_cu;
     const int64_t lanes = static_cast<int64_t>(A->grid_b) * 256;
     // first guess of the checkpoint pool: single frames rays of up to 512
-    // accepted steps (a frame at rtol 1e-10 averages ~300), batches 256; a
-    // forward that finds it short tells the first backward after it, which
-    // grows the pool to the need
-    const int64_t rec = (batch ? 256 : 512) / K + 1;
+    // accepted steps (a frame at rtol 1e-10 averages ~300), batches 256 (with
+    // K = 1, a record per step: 320 and 160 steps); a forward that finds it
+    // short tells the first backward after it, which grows the pool to the need
+    const int64_t rec = K == 1 ? (batch ? 160 : 320) : (batch ? 256 : 512) / K + 1;
     const int64_t per_ray = (rec + (int64_t(1) << A->ck_lg) - 1) >> A->ck_lg;
     const int64_t max_chunks = std::min<int64_t>(kMaxPoolBytes / static_cast<int64_t>(A->chunk_bytes()), 0xffffffffLL);
     if (n_rays > max_chunks) return bail(bhx_fail(ctx, BHX_E_TOO_LARGE, "checkpoint pool above 128 GiB: raise K"));
@@ -746,13 +762,17 @@ int bhx_adjoint_autotune(bhx_adjoint* A, const double* w, const int32_t* Ks, int
         if (Ks[i] < 1 || Ks[i] > 4096) return bhx_fail(ctx, BHX_E_INVALID, "checkpoint interval K must be in [1, 4096]");
     int K_best = A->K;
     double ms_best = 0.0;
+    bool any = false;
     std::vector<double> grad(4 * A->scenes.size());
     for (int i = 0; i < n_K; ++i) {
+        // a K whose records would not fit the pool's cap is no candidate
         int rc = adj_set_K(A, Ks[i]);
+        if (rc == BHX_E_TOO_LARGE) continue;
         if (rc) return rc;
         // one untimed pass (grows the checkpoint pool if this K needs more)
         rc = adj_record(A, nullptr, st);
         if (!rc) rc = adj_grow_if_short(A, st);
+        if (rc == BHX_E_TOO_LARGE) continue;
         if (!rc) rc = adj_sweep(A, w, grad.data(), st);
         if (rc) return rc;
         const auto t0 = std::chrono::steady_clock::now();
@@ -762,10 +782,15 @@ int bhx_adjoint_autotune(bhx_adjoint* A, const double* w, const int32_t* Ks, int
         }
         if (rc) return rc;
         const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / reps;
-        if (i == 0 || ms < ms_best) {
+        if (!any || ms < ms_best) {
             ms_best = ms;
             K_best = Ks[i];
+            any = true;
         }
+    }
+    if (!any) {
+        (void)adj_set_K(A, K_best);
+        return bhx_fail(ctx, BHX_E_TOO_LARGE, "no candidate checkpoint interval fits the 128 GiB pool");
     }
     const int rc = adj_set_K(A, K_best);
     if (rc) return rc;

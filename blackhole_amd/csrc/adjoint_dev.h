@@ -35,7 +35,13 @@ using namespace dp5;
 
 constexpr double kPi_ = 3.14159265358979323846;
 constexpr double kSigmaSB_ = 5.670374419e-8;
-constexpr int kCk = 8;   // checkpoint record: y[6], h, controller memory (beta log err_prev)
+// A checkpoint record: y[6], h, controller memory (beta log err_prev), every
+// K accepted steps.  With K = 1 the forward keeps a record of every accepted
+// step instead - (r, theta, p_r, p_theta) at its start, the step size it was
+// accepted with and its near-jet flag - and the backward reads them without
+// replaying.
+constexpr int kCk = 8;
+constexpr int kStep = 6;
 
 // Per-frame constants of a differentiable render.
 struct Frame {
@@ -79,14 +85,34 @@ __host__ __device__ inline void write_ck(double* ck, const Ray& s) {
     ck[7] = s.lpw;
 }
 
+// The record of one accepted step (K = 1): the state it started from (t and
+// phi never feed back), the step size it was accepted with, whether the jet
+// was sampled on it.
+__host__ __device__ inline void write_step(double* ck, const Ray& s, bool near) {
+    ck[0] = s.y[1];
+    ck[1] = s.y[2];
+    ck[2] = s.y[4];
+    ck[3] = s.y[5];
+    ck[4] = s.h;
+    ck[5] = near ? 1.0 : 0.0;
+}
+// Position of a record in a store walked from the last record back.
+struct CkPos {
+    int32_t c, j;   // chunk (stores that have them), record
+};
+
 // A ray's checkpoints in one flat array: record j at base + j kCk.  The
 // forward and the backward take any store with put(j) (the slot to write
-// record j to, or NULL to drop it; records come in order) and get(j); the
+// record j to, or NULL to drop it; records come in order), get(j), and the
+// walk back from the last of n records, rpos(n) / at(p) / step_back(p); the
 // kernels use a chunked pool (adjoint.hip).
 struct FlatCk {
     double* base;
     __host__ __device__ double* put(int j) { return base + static_cast<int64_t>(j) * kCk; }
     __host__ __device__ const double* get(int j) const { return base + static_cast<int64_t>(j) * kCk; }
+    __host__ __device__ CkPos rpos(int n) const { return {0, n - 1}; }
+    __host__ __device__ const double* at(const CkPos& p) const { return get(p.j); }
+    __host__ __device__ void step_back(CkPos& p) const { --p.j; }
 };
 
 // One DP5 attempt from s (the tracer's arithmetic, src/geodesic.h:130-169).
@@ -786,7 +812,8 @@ __host__ __device__ inline void fwd_init(const Frame& F, const FrameCam& cam, in
     s.jet_wait = 0;
     s.nsteps = 0;
     s.tries = 0;
-    if (double* r = ck.put(0)) write_ck(r, s);
+    if (F.K > 1)
+        if (double* r = ck.put(0)) write_ck(r, s);
 }
 
 // Dense-output coefficients of the components (r, theta, p_r, p_theta).
@@ -817,6 +844,9 @@ __host__ __device__ inline int fwd_step(const Frame& F, Ray& s, Ck& ck, double y
     if (accepted(A)) {
         ++s.nsteps;
         s.tries = 0;
+        const bool near = near_jet(F, s, A);
+        if (F.K == 1)
+            if (double* r = ck.put(s.nsteps - 1)) write_step(r, s, near);
         int term = -1;
         double s_jet = 1.0;
         const bool horizon = A.yn[1] <= F.r_stop;                    // render.cpp:172-177
@@ -842,7 +872,7 @@ __host__ __device__ inline int fwd_step(const Frame& F, Ray& s, Ck& ck, double y
                 term = BHX_DISC;
             }
         }
-        if (near_jet(F, s, A)) {
+        if (near) {
             double cf[4][5];
             dense4(s, A, cf);
             if (pend) {
@@ -864,7 +894,7 @@ __host__ __device__ inline int fwd_step(const Frame& F, Ray& s, Ck& ck, double y
             if (s.nsteps >= F.max_steps) {
                 for (int c = 0; c < 6; ++c) yo[c] = s.y[c];
                 term = BHX_EXHAUSTED;
-            } else if (s.nsteps % F.K == 0) {
+            } else if (F.K > 1 && s.nsteps % F.K == 0) {
                 if (double* r = ck.put(s.nsteps / F.K)) write_ck(r, s);
             }
         }
@@ -926,6 +956,110 @@ __host__ __device__ inline bool same_ck(const Ray& s, const double* ck) {
     return s.h == ck[6] && s.lpw == ck[7];
 }
 
+// The camera end of a ray's adjoint: x0 = (r_cam, theta_cam, p_r, p_theta)
+// and (p_t, p_phi) as functions of (spin, r_cam, theta_cam) through the
+// frame's tetrad and metric; adds d(w I)/d(spin, theta_cam, eddington, r_cam)
+// to grad from the adjoints xb of x0 and cb of (a, p_t, p_phi).
+__host__ __device__ inline void bwd_camera(const Frame& F, const FrameCam& cam, const double d[3], double w,
+                                           double wk, double I_disc, double J, const double xb[4],
+                                           const double cb[3], double grad[4]) {
+    const RaygenArgs& G = cam.G;
+    using D3 = Dual<3>;
+    const double base[10] = {G.e00, G.e03, G.e11, G.e22, G.e33, G.gl_tt, G.gl_tp, G.gl_rr, G.gl_hh, G.gl_pp};
+    D3 e[10];
+    for (int i = 0; i < 10; ++i) {
+        e[i] = D3(base[i]);
+        for (int m = 0; m < 3; ++m) e[i].d[m] = cam.dcam[i][m];
+    }
+    D3 pt, pr, pth, pph;
+    camera_photon_t<D3>(e[0], e[1], e[2], e[3], e[4], e[5], e[6], e[7], e[8], e[9], d, pt, pr, pth, pph);
+    double dp[3];
+    for (int m = 0; m < 3; ++m)
+        dp[m] = xb[2] * pr.d[m] + xb[3] * pth.d[m] + cb[1] * pt.d[m] + cb[2] * pph.d[m];
+    const double I = I_disc + F.kappa * J;
+    grad[0] += cb[0] + dp[0] + wk * J * F.dlnj0_da;   // spin (incl. the jet power's)
+    grad[1] += xb[1] + dp[2];                         // theta_cam
+    grad[2] += w * I / F.eddington;                   // eddington: disc flux and jet power ~ Mdot
+    grad[3] += xb[0] + dp[1];                         // r_cam
+}
+
+// The adjoint of one ray from a record of every accepted step (K = 1): no
+// replay.  The records hold what a replay would give - each step's start
+// state, accepted step size and near-jet flag, written by the same forward
+// arithmetic - so the result equals bwd_ray's at any other K bit for bit.
+// A disc hit's terminal step is taken again from its record for the crossing
+// fraction and radius.  Returns 1 when that step does not reproduce the
+// forward's crossing (0 otherwise).
+template <class Ck>
+__host__ __device__ inline int bwd_ray_log(const Frame& F, const FrameCam& cam, int64_t ray, int n_acc, int cls,
+                                           double w, const Ck& ck, double grad[4], double* lds = nullptr) {
+    const RaygenArgs& G = cam.G;
+    int mismatch = 0;
+    int px, py, k;
+    ray_pixel(G, ray, px, py, k);
+    double d[3];
+    camera_dir(G, px, py, k, d);
+    Ray s;
+    {
+        double pr, pth;
+        camera_photon(G, d, s.pt, pr, pth, s.pph);
+    }
+    const RayK K = ray_k(F.a, F.a2, s.pt, s.pph);
+    const double wk = w * F.kappa;
+    const bool disc = cls == BHX_DISC;
+    double xb[4] = {0.0, 0.0, 0.0, 0.0};
+    double cb[3] = {0.0, 0.0, 0.0};
+    double I_disc = 0.0, J = 0.0, se = 0.5, Or = 0.0;
+    CkPos p = ck.rpos(n_acc);
+    if (disc) {
+        // t and phi (not recorded) enter only the step's error norm, not its
+        // r and theta: the crossing comes out as in the forward
+        const double* o = ck.at(p);
+        s.y[0] = 0.0; s.y[1] = o[0]; s.y[2] = o[1]; s.y[3] = 0.0; s.y[4] = o[2]; s.y[5] = o[3];
+        s.h = o[4];
+        Rhs f;
+        kerr_rhs(K, s.y[1], s.y[2], s.y[4], s.y[5], f);
+        for (int c = 0; c < 6; ++c) s.k1[c] = f.d[c];
+        s.c_prev = f.cs;
+        Att A;
+        attempt(K, s, F.rtol, F.atol, A);
+        double cr[5], ct[5];
+        dense_coeffs(s, A, 1, cr);
+        dense_coeffs(s, A, 2, ct);
+        se = root_theta(ct[0], ct[1], ct[2], ct[3], ct[4], s.c_prev, s.y[2], A.yn[2]);
+        const double r_hit = dense_comp(cr[0], cr[1], cr[2], cr[3], cr[4], se);
+        if (!(s.c_prev * A.g7.cs < 0.0) || !(r_hit >= F.r_in && r_hit <= F.r_out)) ++mismatch;
+        I_disc = disc_intensity(F, r_hit, s.pt, s.pph);
+        if (I_disc > 0.0) {
+            double dI[4];
+            disc_intensity_grad(F, r_hit, s.pt, s.pph, dI);
+            Or = w * dI[0];
+            cb[0] += w * dI[1]; cb[1] += w * dI[2]; cb[2] += w * dI[3];
+        }
+    }
+    // sweep back over the records, the terminal step first; record i - 1 is
+    // loaded while step i is swept
+    double nx[kSeg];
+    {
+        const double* o = ck.at(p);
+        for (int c = 0; c < kSeg; ++c) nx[c] = o[c];
+    }
+    for (int i = n_acc - 1; i >= 0; --i) {
+        const double xi[4] = {nx[0], nx[1], nx[2], nx[3]};
+        const double hi = nx[4];
+        const bool jet_i = nx[5] != 0.0;
+        if (i > 0) {
+            ck.step_back(p);
+            const double* o = ck.at(p);
+            for (int c = 0; c < kSeg; ++c) nx[c] = o[c];
+        }
+        const bool term = disc && i == n_acc - 1;
+        step_reverse(F, K, xi, hi, jet_i, term, se, Or, wk, xb, cb, &J, lds);
+    }
+    bwd_camera(F, cam, d, w, wk, I_disc, J, xb, cb, grad);
+    return mismatch;
+}
+
 // Adjoint of w * I for one ray that ended after n_acc accepted steps with
 // class cls, I = sigma (g T)^4 / pi of a disc hit + kappa J of the jet:
 // adds d(w I)/d(spin, theta_cam, eddington, r_cam) to grad.  Returns the
@@ -935,6 +1069,7 @@ template <class Ck>
 __host__ __device__ inline int bwd_ray(const Frame& F, const FrameCam& cam, int64_t ray, int n_acc, int cls,
                                        double w, const Ck& ck, double* seg, int64_t stride, double grad[4],
                                        double* lds = nullptr) {
+    if (F.K == 1) return bwd_ray_log(F, cam, ray, n_acc, cls, w, ck, grad, lds);
     const RaygenArgs& G = cam.G;
     int mismatch = 0;
     int px, py, k;
@@ -1021,25 +1156,7 @@ __host__ __device__ inline int bwd_ray(const Frame& F, const FrameCam& cam, int6
             step_reverse(F, K, xi, hi, jet_i, term, se, Or, wk, xb, cb, &J, lds);
         }
     }
-    // camera: x0 = (r_cam, theta_cam, p_r, p_theta) and (p_t, p_phi) as
-    // functions of (spin, r_cam, theta_cam) through the frame's tetrad and metric
-    using D3 = Dual<3>;
-    const double base[10] = {G.e00, G.e03, G.e11, G.e22, G.e33, G.gl_tt, G.gl_tp, G.gl_rr, G.gl_hh, G.gl_pp};
-    D3 e[10];
-    for (int i = 0; i < 10; ++i) {
-        e[i] = D3(base[i]);
-        for (int m = 0; m < 3; ++m) e[i].d[m] = cam.dcam[i][m];
-    }
-    D3 pt, pr, pth, pph;
-    camera_photon_t<D3>(e[0], e[1], e[2], e[3], e[4], e[5], e[6], e[7], e[8], e[9], d, pt, pr, pth, pph);
-    double dp[3];
-    for (int m = 0; m < 3; ++m)
-        dp[m] = xb[2] * pr.d[m] + xb[3] * pth.d[m] + cb[1] * pt.d[m] + cb[2] * pph.d[m];
-    const double I = I_disc + F.kappa * J;
-    grad[0] += cb[0] + dp[0] + wk * J * F.dlnj0_da;   // spin (incl. the jet power's)
-    grad[1] += xb[1] + dp[2];                         // theta_cam
-    grad[2] += w * I / F.eddington;                   // eddington: disc flux and jet power ~ Mdot
-    grad[3] += xb[0] + dp[1];                         // r_cam
+    bwd_camera(F, cam, d, w, wk, I_disc, J, xb, cb, grad);
     return mismatch;
 }
 

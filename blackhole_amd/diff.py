@@ -112,7 +112,7 @@ class DiscRender:
             self.tr._stream(None)), "bhx_adjoint_copy_rays")
         return out
 
-    def autotune(self, w, Ks=(16, 32, 64, 128, 256), reps=2):
+    def autotune(self, w, Ks=(1, 16, 32, 64, 128, 256), reps=2):
         """Pick the fastest checkpoint interval among Ks for this frame (timed
         gradient passes with dL/dI = w); returns (K, ms per pass).  The next
         backward needs a new forward."""

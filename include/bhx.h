@@ -327,7 +327,10 @@ int bhx_batch_autotune(bhx_batch *batch, int32_t reps, bhx_launch *best, double 
  * forward again before it sweeps; the backward replays each K-step segment from its checkpoint with
  * the same arithmetic and sweeps it in reverse with the step sizes frozen,
  * and differentiates the disc crossing time by the implicit-function
- * theorem.  No counterpart in the reference, which has no derivatives; the
+ * theorem.  K = 1 records every accepted step instead (48 bytes: its start
+ * (r, theta, p_r, p_theta), step size and jet flag; chunks of 128 records),
+ * about steps x 48 bytes per ray, and the backward reads the records without
+ * replaying; the gradient is the same, bit for bit, at every K.  No counterpart in the reference, which has no derivatives; the
  * forward image is the reference's (same tracer).  Scenes with a neutron
  * star, the reference's jets (jet_model 0), the Sun or a hot spot are
  * rejected.
@@ -356,7 +359,9 @@ int bhx_adjoint_copy_rays(bhx_adjoint *adj, uint8_t *cls, double *y, int32_t *st
  * forward + backward with dL/dI = w, device) at the adjoint's current scene;
  * leaves the fastest K set and returns it and its time per pass [ms].  The
  * gradient does not depend on K (the replay reproduces every checkpoint).
- * Synchronous; the next backward needs a new forward. */
+ * A candidate whose records would exceed the pool's 128 GiB cap is skipped
+ * (BHX_E_TOO_LARGE when none fits).  Synchronous; the next backward needs a
+ * new forward. */
 int bhx_adjoint_autotune(bhx_adjoint *adj, const double *w, const int32_t *Ks, int32_t n_K, int32_t reps,
                          int32_t *best_K, double *best_ms, void *stream);
 /* The checkpoint interval in use. */

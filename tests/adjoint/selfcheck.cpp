@@ -12,6 +12,7 @@
 #include "../../blackhole_amd/csrc/adjoint_frame.h"
 
 #include <cstdio>
+#include <cstring>
 #include <random>
 #include <vector>
 
@@ -257,6 +258,46 @@ static void check_ray(const char* name, const bhx_scene& sc, int K, int64_t stri
     std::printf("%s %d %.3e mismatch %d jet %d\n", name, n, worst, mism, n_jet);
 }
 
+// A record of every accepted step (K = 1, read back without replaying) and
+// segments of K accepted steps replayed from checkpoints give the same
+// adjoint bit for bit: the same rays (class, steps, intensity) and, per lit
+// ray, the same four derivatives.  Prints the rays compared and how many
+// differ.
+static void check_log_equals_replay(const char* name, const bhx_scene& sc, int K, int64_t stride, int n_max) {
+    Frame F1, FK;
+    FrameCam G1, GK;
+    if (!make_frame(sc, 1, F1, G1) || !make_frame(sc, K, FK, GK)) {
+        std::printf("%s 0 1 mismatch 0\n", name);
+        return;
+    }
+    std::vector<double> ck1(static_cast<size_t>(F1.max_steps + 1) * kCk);
+    std::vector<double> ckK(static_cast<size_t>(FK.max_steps / K + 1) * kCk), seg(static_cast<size_t>(K + 1) * kSeg);
+    FlatCk st1{ck1.data()}, stK{ckK.data()};
+    int n = 0, differ = 0, mism = 0;
+    for (int64_t ray = 0; ray < (int64_t)sc.width * sc.height && n < n_max; ray += stride) {
+        Ray a, b;
+        double ya[6], yb[6], sa = 0.0, sb = 0.0;
+        fwd_init(F1, G1, ray, a, st1);
+        fwd_init(FK, GK, ray, b, stK);
+        int ta = -1, tb = -1;
+        while (ta < 0) ta = fwd_step(F1, a, st1, ya, &sa);
+        while (tb < 0) tb = fwd_step(FK, b, stK, yb, &sb);
+        const double Ia = (ta == BHX_DISC ? disc_intensity(F1, ya[1], a.pt, a.pph) : 0.0) + F1.kappa * a.jet;
+        const double Ib = (tb == BHX_DISC ? disc_intensity(FK, yb[1], b.pt, b.pph) : 0.0) + FK.kappa * b.jet;
+        if (ta != tb || a.nsteps != b.nsteps || std::memcmp(&Ia, &Ib, sizeof Ia) != 0) {
+            ++differ;
+            continue;
+        }
+        if (!(Ia > 0.0)) continue;
+        double ga[4] = {0, 0, 0, 0}, gb[4] = {0, 0, 0, 0};
+        mism += bwd_ray(F1, G1, ray, a.nsteps, ta, 1.0, st1, seg.data(), 1, ga);
+        mism += bwd_ray(FK, GK, ray, b.nsteps, tb, 1.0, stK, seg.data(), 1, gb);
+        if (std::memcmp(ga, gb, sizeof ga) != 0) ++differ;
+        ++n;
+    }
+    std::printf("%s %d %d mismatch %d\n", name, n, differ, mism);
+}
+
 // The differentiation scene, sampled: a = 0.7, i = 45 deg, r = 500 M, +-20 M.
 static bhx_scene disc_scene(double rtol, double atol) {
     bhx_scene sc{};
@@ -295,5 +336,9 @@ int main() {
     check_ray("ray_rtol1e-12_K7", disc_scene(1e-12, 1e-14), 7, 37, 40);
     check_ray("jet_i10_K16", jet_scene(10.0, 120.0, 1e-10, 1e-12), 16, 7, 80);
     check_ray("jet_i3_K7", jet_scene(3.0, 120.0, 1e-12, 1e-14), 7, 13, 80);
+    check_ray("ray_rtol1e-09_K1", disc_scene(1e-9, 1e-11), 1, 37, 40);
+    check_ray("jet_i10_K1", jet_scene(10.0, 120.0, 1e-10, 1e-12), 1, 7, 80);
+    check_log_equals_replay("log_vs_replay_disc", disc_scene(1e-10, 1e-12), 16, 29, 60);
+    check_log_equals_replay("log_vs_replay_jet", jet_scene(10.0, 120.0, 1e-10, 1e-12), 7, 11, 60);
     return 0;
 }

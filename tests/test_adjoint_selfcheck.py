@@ -77,3 +77,17 @@ def test_whole_rays_with_conical_jet(selfcheck):
     r = selfcheck["jet_i3_K7"]
     assert r["n"] >= 40 and r["jet"] >= 10 and r["mismatch"] == 0, r
     assert r["err"] < 1e-5, r
+
+
+def test_step_records_without_replay(selfcheck):
+    # K = 1: the forward keeps a record of every accepted step and the
+    # backward reads them instead of replaying segments; against central
+    # differences as above, and bit for bit the gradient of the replaying
+    # backward (K = 16, K = 7) on the same rays
+    r = selfcheck["ray_rtol1e-09_K1"]
+    assert r["n"] >= 20 and r["mismatch"] == 0 and r["err"] < 1e-4, r
+    r = selfcheck["jet_i10_K1"]
+    assert r["n"] >= 40 and r["jet"] >= 20 and r["mismatch"] == 0 and r["err"] < 1e-4, r
+    for k in ("log_vs_replay_disc", "log_vs_replay_jet"):
+        r = selfcheck[k]
+        assert r["n"] >= 20 and r["err"] == 0 and r["mismatch"] == 0, (k, r)
