@@ -22,7 +22,7 @@
 //                   jet samples' node evaluations with the other lanes); each
 //                   lane's (K + 1)-step segment buffer is a strided HBM
 //                   scratch.  Each ray's contribution is kept and summed per
-//                   frame in a fixed order (k_adj_frame_sums).
+//                   frame in a fixed order (k_adj_slice_sums, k_adj_frame_sums).
 #include "adjoint_frame.h"
 #include "context.h"
 
@@ -322,15 +322,20 @@ __global__ void __launch_bounds__(256) k_adj_backward(const AdjConst* __restrict
 }
 
 // Per frame (of a batch, or the one frame), the sum of its processed rays'
-// contributions (lit rays with dL/dI != 0), in a fixed order: one block per
-// frame, each thread a fixed stride of the frame's rays, then a fixed tree;
-// deterministic.
-__global__ void __launch_bounds__(256) k_adj_frame_sums(const AdjArgs P, const double* __restrict__ w,
-                                                        double* __restrict__ out) {
+// contributions (lit rays with dL/dI != 0) in a fixed order, in two passes:
+// the frame's rays in kSumSlices contiguous slices, one block each (each
+// thread a fixed stride of the slice, then a fixed tree), then per frame the
+// slices' sums in slice order.  Deterministic, and the same order for a
+// frame alone as inside a batch.
+constexpr int kSumSlices = 64;
+__global__ void __launch_bounds__(256) k_adj_slice_sums(const AdjArgs P, const double* __restrict__ w,
+                                                        double* __restrict__ part) {
     __shared__ double red[4][256];
-    const int64_t base = static_cast<int64_t>(blockIdx.x) * P.n_per;
+    const int64_t frame = blockIdx.y, slice = blockIdx.x;
+    const int64_t base = frame * P.n_per;
+    const int64_t lo = P.n_per * slice / kSumSlices, hi = P.n_per * (slice + 1) / kSumSlices;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int64_t r = threadIdx.x; r < P.n_per; r += blockDim.x) {
+    for (int64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) {
         const int64_t ray = base + r;
         if (P.key[ray] > 0 && w[ray] != 0.0)
             for (int c = 0; c < 4; ++c) acc[c] += P.contrib[c * P.n_rays + ray];
@@ -342,7 +347,16 @@ __global__ void __launch_bounds__(256) k_adj_frame_sums(const AdjArgs P, const d
             for (int c = 0; c < 4; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + half];
         __syncthreads();
     }
-    if (threadIdx.x < 4) out[blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x < 4) part[(frame * kSumSlices + slice) * 4 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_adj_frame_sums(const double* __restrict__ part, int64_t n_frames, double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;   // frame * 4 + component
+    if (i >= n_frames * 4) return;
+    const int64_t frame = i / 4, c = i % 4;
+    double sum = 0.0;
+    for (int sl = 0; sl < kSumSlices; ++sl) sum += part[(frame * kSumSlices + sl) * 4 + c];
+    out[i] = sum;
 }
 
 }  // namespace
@@ -358,7 +372,7 @@ struct bhx_adjoint {
     int64_t n_rays = 0;              // samples of all frames
     AdjConst host{};
     DevBuf cst, cams, pool, link, ck_last, ck_nrec, cls, y, steps, I, key, key_sorted, iota, order, sort_tmp, seg,
-        counters, qmap, contrib, frame_grad;
+        counters, qmap, contrib, frame_grad, slice_sums;
     uint32_t pool_cap = 0;           // chunks in the checkpoint pool (>= n_rays: a first chunk per ray)
     int32_t q_chunk = 32;            // the forward's queue: slots per wave take
     int32_t refill = 1;              // idle lanes a wave waits for before it refills them
@@ -521,7 +535,11 @@ int adj_sweep(bhx_adjoint* A, const double* w, double* grad, hipStream_t st) {
         hipLaunchKernelGGL(k_adj_backward<false>, dim3(A->grid_b), dim3(256), 0, st,
                            static_cast<const AdjConst*>(A->cst.p), P);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_adj_frame_sums, dim3(n_frames), dim3(256), 0, st, P, w,
+    hipLaunchKernelGGL(k_adj_slice_sums, dim3(kSumSlices, n_frames), dim3(256), 0, st, P, w,
+                       static_cast<double*>(A->slice_sums.p));
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_adj_frame_sums, dim3((4 * n_frames + 255) / 256), dim3(256), 0, st,
+                       static_cast<const double*>(A->slice_sums.p), static_cast<int64_t>(n_frames),
                        static_cast<double*>(A->frame_grad.p));
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(A->ev[3], st));
@@ -621,7 +639,8 @@ int adj_create(bhx_context* ctx, const bhx_scene* s, int32_t n, int32_t K, bool 
         A->iota.ensure(sizeof(int32_t) * n_rays) != hipSuccess ||
         A->seg.ensure(sizeof(double) * kSeg * (K + 1) * lanes) != hipSuccess ||
         A->contrib.ensure(sizeof(double) * 4 * n_rays) != hipSuccess ||
-        A->frame_grad.ensure(sizeof(double) * 4 * n) != hipSuccess || A->counters.ensure(64) != hipSuccess)
+        A->frame_grad.ensure(sizeof(double) * 4 * n) != hipSuccess ||
+        A->slice_sums.ensure(sizeof(double) * 4 * kSumSlices * n) != hipSuccess || A->counters.ensure(64) != hipSuccess)
         return bail(bhx_fail(ctx, BHX_E_NOMEM, "adjoint buffers"));
     for (auto& e : A->ev)
         if (hipEventCreate(&e) != hipSuccess) return bail(bhx_fail(ctx, BHX_E_HIP, "hipEventCreate"));
@@ -688,7 +707,7 @@ int bhx_adjoint_destroy(bhx_adjoint* A) {
     (void)hipDeviceSynchronize();
     for (DevBuf* d : {&A->cst, &A->cams, &A->pool, &A->link, &A->ck_last, &A->ck_nrec, &A->cls, &A->y, &A->steps,
                       &A->I, &A->key, &A->key_sorted, &A->iota, &A->order, &A->sort_tmp, &A->seg,
-                      &A->counters, &A->qmap, &A->contrib, &A->frame_grad})
+                      &A->counters, &A->qmap, &A->contrib, &A->frame_grad, &A->slice_sums})
         d->release();
     for (auto& e : A->ev)
         if (e) (void)hipEventDestroy(e);
