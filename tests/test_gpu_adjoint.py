@@ -187,6 +187,34 @@ def test_checkpoint_pool_grows_to_the_frame(tracer):
     dr.close()
 
 
+def test_step_records_pool_grows_to_the_frame(tracer):
+    """K = 1 at rtol 1e-12: the rays' step records outgrow the first guess of
+    the pool (320 per ray); the first backward grows it to the forward's count
+    and records again.  The gradient equals the replaying backward's (K = 16)
+    bit for bit, and the pool holds exactly max(1, ceil(steps / 128)) chunks
+    of 128 records of 48 B (plus a 4-byte link) per ray."""
+    import torch
+    sc = S.disc_frame(40, 40, rtol=1e-12, atol=1e-14)
+    w = torch.linspace(-1.0, 1.0, sc.n_rays, dtype=torch.float64, device=f"cuda:{tracer.device}")
+    ref = DiscRender(tracer, sc, K=16)
+    ref.forward_raw(P0)
+    g16 = ref.backward_raw(w)
+    ref.close()
+    dr = DiscRender(tracer, sc, K=1)
+    dr.forward_raw(P0)
+    before = dr.info()["ck_bytes"]
+    g1 = dr.backward_raw(w)
+    info = dr.info()
+    steps = dr.rays()["steps"].astype(np.int64)
+    print(f"[pool K=1] mean steps {steps.mean():.0f}, pool {before} -> {info['ck_bytes']} bytes", flush=True)
+    assert info["ck_bytes"] > before and info["mismatches"] == 0
+    assert np.array_equal(g1, g16), (g1, g16)
+    chunk = 128 * 48 + 4
+    assert info["ck_bytes"] % chunk == 0
+    assert info["ck_bytes"] // chunk == int(np.sum(np.maximum(1, (steps + 127) // 128)))
+    dr.close()
+
+
 def test_full_frame_at_K16(tracer):
     """The 1024^2 differentiation frame at K = 16 (a fixed max_steps / K + 1
     slab per ray would be 84 GB): the pool is sized by the rays' steps."""
