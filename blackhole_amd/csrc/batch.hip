@@ -176,22 +176,44 @@ int bhx_batch_create(bhx_context* ctx, const bhx_scene* scenes, int32_t n, bhx_b
     std::stable_sort(b->queue_order.begin(), b->queue_order.end(),
                      [&](int x, int y) { return rays_of[x] > rays_of[y]; });
 
-    // tile queues of every scene, concatenated in queue order
+    // tile queues of every scene (each centre first: the photon ring's long
+    // rays, then the sky's short ones), merged by their position in their
+    // own scene's queue: the batch's queue takes every scene's centre tiles
+    // first and ends on short rays, with no scene's long rays left for the end
     std::vector<int32_t> order_all, start_all;
     std::vector<size_t> toff(n), soff(n);
     std::vector<int> ntiles(n);
-    int32_t q = 0;
-    for (int i : b->queue_order) {
+    struct QTile {
+        double pos;   // (t + 1/2) / tiles of the scene
+        int rank, scene, t;
+    };
+    std::vector<QTile> merged;
+    for (int r = 0; r < n; ++r) {
+        const int i = b->queue_order[r];
         const bhx_scene& s = scenes[i];
         std::vector<int32_t> order, start;
-        host::tile_queue(s.width, s.height, std::max(1, s.sqrt_spp), q, order, start, host::axis_band(s),
+        host::tile_queue(s.width, s.height, std::max(1, s.sqrt_spp), 0, order, start, host::axis_band(s),
                          s.cam_roll);
         toff[i] = order_all.size();
         soff[i] = start_all.size();
         ntiles[i] = static_cast<int>(order.size());
         order_all.insert(order_all.end(), order.begin(), order.end());
-        start_all.insert(start_all.end(), start.begin(), start.end());
-        q = start.back();
+        start_all.insert(start_all.end(), start.begin(), start.end());   // counts for now
+        for (int t = 0; t < ntiles[i]; ++t) merged.push_back({(t + 0.5) / ntiles[i], r, i, t});
+    }
+    std::stable_sort(merged.begin(), merged.end(), [](const QTile& x, const QTile& y) {
+        return x.pos < y.pos || (x.pos == y.pos && x.rank < y.rank);
+    });
+    {
+        std::vector<int32_t> count(start_all.size());
+        for (int i = 0; i < n; ++i)
+            for (int t = 0; t < ntiles[i]; ++t) count[soff[i] + t] = start_all[soff[i] + t + 1] - start_all[soff[i] + t];
+        int32_t q = 0;
+        for (const QTile& m : merged) {
+            start_all[soff[m.scene] + m.t] = q;
+            q += count[soff[m.scene] + m.t];
+        }
+        for (int i = 0; i < n; ++i) start_all[soff[i] + ntiles[i]] = -1;   // no tile past the last
     }
 
     // device buffers
