@@ -217,7 +217,8 @@ def test_step_records_pool_grows_to_the_frame(tracer):
 
 def test_full_frame_at_K16(tracer):
     """The 1024^2 differentiation frame at K = 16 (a fixed max_steps / K + 1
-    slab per ray would be 84 GB): the pool is sized by the rays' steps."""
+    slab per ray would be 84 GB): the pool is sized by the rays' steps; and
+    at K = 1, the same gradient from the step records."""
     import torch
     sc = S.disc_frame(1024, 1024)
     dr = DiscRender(tracer, sc, K=16)
@@ -231,6 +232,15 @@ def test_full_frame_at_K16(tracer):
     assert info["mismatches"] == 0 and np.all(np.isfinite(g))
     assert info["ck_bytes"] < 8e9
     dr.close()
+    # the same frame at K = 1 (a record of each of its ~260M accepted steps,
+    # read back without replaying): the same gradient, bit for bit
+    d1 = DiscRender(tracer, sc, K=1)
+    d1.forward_raw(P0)
+    g1 = d1.backward_raw(w)
+    i1 = d1.info()
+    print(f"[1024 K1] pool {i1['ck_bytes'] / 1e9:.2f} GB, grad {g1}", flush=True)
+    assert i1["mismatches"] == 0 and np.array_equal(g1, g)
+    d1.close()
 
 
 def test_checkpoint_interval_autotuner(tracer):
