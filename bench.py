@@ -236,9 +236,9 @@ def adjoint_leg(tr, torch, width=512, Ks=(1, 16, 32, 64, 128, 256), rounds=3, fd
     """Differentiable disc image at width^2 (a=0.7, i=45 deg, 0.1 Edd, r=500M;
     target a=0.9, i=50 deg): loss = sum (G*I - G*I_target)^2 with a Gaussian
     sigma = 2 px.  Times a forward render (the plain path: trace + shade); the
-    library's autotuner picks the checkpoint interval K among Ks, and a
-    gradient pass (recording forward + adjoint) is timed at that K and at each
-    candidate; optionally checks the gradient against central differences
+    library's autotuner picks the checkpoint interval K among Ks and then the
+    recording forward's waves per SIMD, and a gradient pass (recording
+    forward + adjoint) is timed at that K and at each candidate; optionally checks the gradient against central differences
     (h = 1e-6 relative) of the oracle's images."""
     import math
     import numpy as np
@@ -274,6 +274,7 @@ def adjoint_leg(tr, torch, width=512, Ks=(1, 16, 32, 64, 128, 256), rounds=3, fd
     dr.forward_raw(P0)
     tuned_K, tuned_ms = dr.autotune(w, Ks=Ks, reps=2)
     res["autotuned_K"] = tuned_K
+    res["autotuned_forward_waves_per_simd"] = dr.info()["fwd_waves"]
     res["autotune_ms_per_pass"] = round(tuned_ms, 3)
 
     def grad_pass():
@@ -293,10 +294,12 @@ def adjoint_leg(tr, torch, width=512, Ks=(1, 16, 32, 64, 128, 256), rounds=3, fd
         info = dr.info()
         res["per_K"][str(K)] = {"grad_pass_ms": round(ms, 3), "record_forward_ms": round(info["fwd_ms"], 3),
                                 "adjoint_ms": round(info["bwd_ms"], 3), "checkpoint_bytes": info["ck_bytes"],
-                                "replay_mismatches": info["mismatches"], "grad_over_fwd": round(ms / fwd_ms, 3)}
+                                "replay_mismatches": info["mismatches"], "forward_waves_per_simd": info["fwd_waves"],
+                                "grad_over_fwd": round(ms / fwd_ms, 3)}
     dr.autotune(w, Ks=(tuned_K,), reps=1)
     ms, g = grad_pass()
     best = (tuned_K, ms, g)
+    res["forward_waves_per_simd"] = dr.info()["fwd_waves"]
     dr.close()
     res["best_K"] = best[0]
     res["grad_pass_ms"] = round(best[1], 3)

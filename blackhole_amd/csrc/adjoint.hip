@@ -377,6 +377,8 @@ struct bhx_adjoint {
     int32_t q_chunk = 32;            // the forward's queue: slots per wave take
     int32_t refill = 1;              // idle lanes a wave waits for before it refills them
     int grid_f = 0, grid_b = 0;
+    int fwd_per_cu = 0;              // recording-forward blocks per CU that fit (one wave per SIMD each)
+    int fwd_waves = 0;               // waves per SIMD the recording forward runs (0: as many as fit)
     double fwd_ms = 0.0, bwd_ms = 0.0;
     bool fwd_pending = false;
     bool have_steps = false;         // steps holds a forward's step counts (the next queue order)
@@ -610,6 +612,7 @@ int adj_create(bhx_context* ctx, const bhx_scene* s, int32_t n, int32_t K, bool 
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adj_forward<false>, 256, 0) != hipSuccess || per_cu < 1)
         per_cu = 1;
+    A->fwd_per_cu = per_cu;
     A->grid_f = per_cu * ctx->n_cu;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adj_backward<false>, 256, 0) != hipSuccess ||
         per_cu < 1)
@@ -811,8 +814,33 @@ int bhx_adjoint_autotune(bhx_adjoint* A, const double* w, const int32_t* Ks, int
         (void)adj_set_K(A, K_best);
         return bhx_fail(ctx, BHX_E_TOO_LARGE, "no candidate checkpoint interval fits the 128 GiB pool");
     }
-    const int rc = adj_set_K(A, K_best);
+    int rc = adj_set_K(A, K_best);
     if (rc) return rc;
+    // then the recording forward's waves per SIMD at that K: a small frame
+    // lasts as long as its longest ray, which fewer waves per SIMD finish
+    // sooner; a large one is throughput-bound and wants them all
+    const int w_start = A->grid_f / ctx->n_cu;
+    int w_best = w_start;
+    for (int wv = A->fwd_per_cu; wv >= 1; --wv) {
+        A->grid_f = wv * ctx->n_cu;
+        rc = adj_record(A, nullptr, st);
+        if (!rc) rc = adj_grow_if_short(A, st);
+        if (!rc) rc = adj_sweep(A, w, grad.data(), st);
+        if (rc) return rc;
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int r = 0; r < reps && !rc; ++r) {
+            rc = adj_record(A, nullptr, st);
+            if (!rc) rc = adj_sweep(A, w, grad.data(), st);
+        }
+        if (rc) return rc;
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / reps;
+        if (ms < ms_best) {
+            ms_best = ms;
+            w_best = wv;
+        }
+    }
+    A->grid_f = w_best * ctx->n_cu;
+    A->fwd_waves = w_best;
     if (best_K) *best_K = K_best;
     if (best_ms) *best_ms = ms_best;
     return BHX_OK;
@@ -831,6 +859,21 @@ int bhx_adjoint_set_launch(bhx_adjoint* A, int32_t q_chunk, int32_t refill) {
     if (refill < 0 || refill > 64) return bhx_fail(ctx, BHX_E_INVALID, "refill must be in [0, 64]");
     if (q_chunk) A->q_chunk = q_chunk;
     if (refill) A->refill = refill;
+    return BHX_OK;
+}
+
+int bhx_adjoint_set_forward_waves(bhx_adjoint* A, int32_t waves) {
+    if (!A) return BHX_E_INVALID;
+    if (waves < 0 || waves > 16) return bhx_fail(A->ctx, BHX_E_INVALID, "forward waves per SIMD must be in [0, 16]");
+    A->fwd_waves = waves;
+    const int w = waves > 0 ? std::min(waves, A->fwd_per_cu) : A->fwd_per_cu;
+    A->grid_f = w * A->ctx->n_cu;
+    return BHX_OK;
+}
+
+int bhx_adjoint_get_forward_waves(const bhx_adjoint* A, int32_t* waves) {
+    if (!A || !waves) return BHX_E_INVALID;
+    *waves = A->grid_f / A->ctx->n_cu;
     return BHX_OK;
 }
 

@@ -114,8 +114,9 @@ class DiscRender:
 
     def autotune(self, w, Ks=(1, 16, 32, 64, 128, 256), reps=2):
         """Pick the fastest checkpoint interval among Ks for this frame (timed
-        gradient passes with dL/dI = w); returns (K, ms per pass).  The next
-        backward needs a new forward."""
+        gradient passes with dL/dI = w), then the recording forward's waves per
+        SIMD at that K; returns (K, ms per pass).  The next backward needs a
+        new forward."""
         w = w.contiguous()
         arr = (C.c_int32 * len(Ks))(*Ks)
         k, ms = C.c_int32(), C.c_double()
@@ -131,13 +132,21 @@ class DiscRender:
         native.check(self.tr.ctx, self.lib.bhx_adjoint_set_launch(self.adj, int(q_chunk), int(refill)),
                      "bhx_adjoint_set_launch")
 
+    def set_forward_waves(self, waves=0):
+        """Waves per SIMD of the recording forward (0: as many as fit)."""
+        native.check(self.tr.ctx, self.lib.bhx_adjoint_set_forward_waves(self.adj, int(waves)),
+                     "bhx_adjoint_set_forward_waves")
+
     def info(self):
         ck, disc, mism = C.c_int64(), C.c_int64(), C.c_int64()
         f, b = C.c_double(), C.c_double()
+        wv = C.c_int32()
         native.check(self.tr.ctx, self.lib.bhx_adjoint_info(self.adj, C.byref(ck), C.byref(disc), C.byref(mism),
                                                             C.byref(f), C.byref(b)), "bhx_adjoint_info")
+        native.check(self.tr.ctx, self.lib.bhx_adjoint_get_forward_waves(self.adj, C.byref(wv)),
+                     "bhx_adjoint_get_forward_waves")
         return {"ck_bytes": ck.value, "lit_rays": disc.value, "mismatches": mism.value,
-                "fwd_ms": f.value, "bwd_ms": b.value}
+                "fwd_ms": f.value, "bwd_ms": b.value, "fwd_waves": wv.value}
 
     # ---- autograd ------------------------------------------------------------
     def __call__(self, p):

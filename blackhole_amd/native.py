@@ -24,7 +24,7 @@ SYMBOLS = (
     "bhx_adjoint_rays", "bhx_adjoint_copy_rays", "bhx_adjoint_info", "bhx_png_encode", "bhx_png_encode_dev",
     "bhx_apng_encode", "bhx_apng_encode_dev", "bhx_bytes_free", "bhx_trace_noalloc_dev", "bhx_uniform_sky",
     "bhx_adjoint_autotune", "bhx_adjoint_get_interval", "bhx_adjoint_create_batch", "bhx_adjoint_backward_batch",
-    "bhx_adjoint_set_launch",
+    "bhx_adjoint_set_launch", "bhx_adjoint_set_forward_waves", "bhx_adjoint_get_forward_waves",
 )
 
 
@@ -78,6 +78,8 @@ def lib():
     L.bhx_adjoint_create_batch.argtypes = [P, C.POINTER(Scene), C.c_int32, C.c_int32, C.POINTER(P)]
     L.bhx_adjoint_backward_batch.argtypes = [P, P, P, P]
     L.bhx_adjoint_set_launch.argtypes = [P, C.c_int32, C.c_int32]
+    L.bhx_adjoint_set_forward_waves.argtypes = [P, C.c_int32]
+    L.bhx_adjoint_get_forward_waves.argtypes = [P, C.POINTER(C.c_int32)]
     L.bhx_adjoint_set_scene.argtypes = [P, C.POINTER(Scene)]
     L.bhx_adjoint_forward.argtypes = [P, P, P]
     L.bhx_adjoint_backward.argtypes = [P, P, P, P]

@@ -357,8 +357,10 @@ int bhx_adjoint_copy_rays(bhx_adjoint *adj, uint8_t *cls, double *y, int32_t *st
 /* Autotuner of the checkpoint interval: for each of the n_K candidates Ks
  * (at most 16), one untimed and `reps` timed gradient passes (recording
  * forward + backward with dL/dI = w, device) at the adjoint's current scene;
- * leaves the fastest K set and returns it and its time per pass [ms].  The
- * gradient does not depend on K (the replay reproduces every checkpoint).
+ * leaves the fastest K set, then times the recording forward at each number
+ * of waves per SIMD that fits and keeps the fastest too, and returns K and
+ * the time per pass [ms] of the pair.  The gradient does not depend on
+ * either (the replay reproduces every checkpoint).
  * A candidate whose records would exceed the pool's 128 GiB cap is skipped
  * (BHX_E_TOO_LARGE when none fits).  Synchronous; the next backward needs a
  * new forward. */
@@ -372,6 +374,12 @@ int bhx_adjoint_get_interval(const bhx_adjoint *adj, int32_t *K);
  * (a single frame), 32 and 4 (a batch).
  * Results do not depend on them. */
 int bhx_adjoint_set_launch(bhx_adjoint *adj, int32_t q_chunk, int32_t refill);
+/* Waves per SIMD the recording forward runs (1 up to what fits, 0: what
+ * fits, the default).  A frame of few rays per lane lasts as long as its
+ * longest ray, which fewer waves per SIMD finish sooner; bhx_adjoint_autotune
+ * picks it after K.  Results do not depend on it. */
+int bhx_adjoint_set_forward_waves(bhx_adjoint *adj, int32_t waves);
+int bhx_adjoint_get_forward_waves(const bhx_adjoint *adj, int32_t *waves);
 /* A batch of n frames of one size and one physics (spin, disc, jet,
  * integrator, cam_r) that differ in their camera only (cam_theta, cam_phi,
  * fov, roll, jitter seed): one persistent forward over all n x samples rays,
