@@ -155,6 +155,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
     bool live = false;
     int q = 0;                          // queue slot of the lane's ray
     double y[6], k1[6];                 // t, r, th, ph, p_r, p_th
+    double pc[3] = {0.0, 0.0, 0.0};     // y's pseudo-Cartesian point (jet cull, Sun): a step's end is the next one's start
     double pt = 0.0, pph = 0.0;         // conserved p_t, p_phi
     double kapt = 0.0, kapph = 0.0, ka2pt2 = 0.0, kpph2 = 0.0;   // their products in RayK
     double h = 0.0, lpw = 0.0, c_prev = 0.0, q0raw = 0.0, iq0 = 0.0;   // lpw = beta log(err_prev)
@@ -321,6 +322,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
                         kpph2 = K0.pph2;
                     }
                     c_prev = I[kI_C0 * N + q];
+                    if (HAS_JET || HAS_SUN) to_cart(la2, y[1], y[2], y[3], pc);
                     q0raw = I[kI_Q0 * N + q];
                     iq0 = rcp(fabs(q0raw) + 1e-12);
                     h = SK(h0);
@@ -476,7 +478,8 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
             bool near_jet = false, sun_cand = false;
             double p0[3], p1[3];
             if (HAS_JET || HAS_SUN) {
-                to_cart(la2, y[1], y[2], y[3], p0);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) p0[c] = pc[c];
                 to_cart(la2, yn[1], yn[2], yn[3], p1);
                 if (HAS_JET) near_jet = seg_dist(p0, p1) < EVS(r_bound);        // render.cpp:151-152
                 // conical jet: only steps whose dense output can reach the cone
@@ -619,6 +622,10 @@ __device__ __forceinline__ void trace_body(const TraceArgs& P, const SceneK* sk,
             if (term < 0) {
 #pragma unroll
                 for (int c = 0; c < 6; ++c) { y[c] = yn[c]; k1[c] = g7.d[c]; }
+                if (HAS_JET || HAS_SUN) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) pc[c] = p1[c];
+                }
                 c_prev = g7.cs;
                 lpw = kBeta * dmax(lg, kLog1em4);   // err_prev = max(err, 1e-4)
                 h *= fac;
