@@ -553,15 +553,16 @@ def main():
         order.append(sk.order_ms)
     trace_ms = sum(kern) / reps
     order_ms = sum(order) / reps
-    # end to end: the frame to 8-bit sRGB on the device (trace, shade,
-    # auto exposure - a radix selection with host round trips - and develop)
+    # end to end: the frame to 8-bit sRGB on the device (trace, shade, then
+    # auto exposure - a radix selection - and develop, the exposure kept on
+    # the device so the frames queue without a host round trip)
     e2e_steps = max(1, min(args.steps, 10))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(e2e_steps):
         tr.trace_async(sc, rays)
         tr.shade_async(sc, rays, out)
-        rgb = tr.develop(out["xyz"], tr.auto_exposure(out["xyz"]))
+        rgb = tr.develop_auto(out["xyz"])
     torch.cuda.synchronize()
     e2e_ms = (time.perf_counter() - t0) / e2e_steps * 1e3
     del rgb

@@ -457,6 +457,27 @@ int bhx_develop_dev(bhx_context* ctx, const double* xyz, int64_t n, double expos
     return BHX_OK;
 }
 
+int bhx_develop_auto_dev(bhx_context* ctx, const double* xyz, int64_t n, double key, int32_t tm,
+                         double log_decades, uint8_t* rgb, double* exposure_dev, void* stream) {
+    if (!ctx) return BHX_E_INVALID;
+    if (!xyz || !rgb || n < 0) return fail(ctx, BHX_E_INVALID, "bad argument");
+    if (tm < BHX_TM_LINEAR || tm > BHX_TM_HDR) return fail(ctx, BHX_E_INVALID, "unknown tone map");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const double log_k = std::pow(10.0, std::clamp(log_decades, 1.0, 20.0));
+    HIPCHK(ctx->exp_hist.ensure(sizeof(unsigned int) * 2 * 4096));
+    HIPCHK(ctx->exp_state.ensure(sizeof(ExposureState)));
+    unsigned int* hist = static_cast<unsigned int*>(ctx->exp_hist.p);
+    ExposureState* state = static_cast<ExposureState*>(ctx->exp_state.p);
+    // the selection of bhx_auto_exposure_dev, then develop reads its state
+    // on the device: the stream never waits for the host
+    HIPCHK(launch_zero_u32(hist, 2 * 4096, st));
+    HIPCHK(launch_zero_u64(reinterpret_cast<unsigned long long*>(state), sizeof(ExposureState) / 8, st));
+    if (n > 0) HIPCHK(launch_exposure(xyz, n, state, hist, st));
+    HIPCHK(launch_develop_auto(xyz, n, state, key, exposure_dev, tm, log_k, rgb, st));
+    return BHX_OK;
+}
+
 int bhx_blackbody_xyz_dev(bhx_context* ctx, const double* T, int64_t n, double* xyz, void* stream) {
     if (!ctx) return BHX_E_INVALID;
     if (!T || !xyz || n < 0) return fail(ctx, BHX_E_INVALID, "bad argument");

@@ -31,6 +31,8 @@ hipError_t trace_batch_occupancy(int block, int n_scenes, bool lean, int* blocks
 hipError_t launch_shade(const ShadeArgs& args, int blocks, hipStream_t s);
 hipError_t launch_develop(const double* xyz, int64_t n, double exposure, int tm, double log_k,
                           uint8_t* rgb, hipStream_t s);
+hipError_t launch_develop_auto(const double* xyz, int64_t n, const ExposureState* state, double key,
+                               double* e_out, int tm, double log_k, uint8_t* rgb, hipStream_t s);
 hipError_t launch_exposure(const double* xyz, int64_t n, ExposureState* state, unsigned int* hist,
                            hipStream_t s);
 hipError_t launch_bb(const double* tab, const double* T, int64_t n, double* out, hipStream_t s);

@@ -363,9 +363,30 @@ __device__ __forceinline__ double srgb_oetf(double v) {
     return (v <= 0.0031308) ? 12.92 * v : 1.055 * pow(v, 1.0 / 2.4) - 0.055;
 }
 
-__global__ void k_develop(const double* __restrict__ xyz, int64_t n, double exposure, int tm,
-                          double log_k, uint8_t* __restrict__ rgb) {
+// The exposure of img::auto_exposure (src/image.cpp:37-44) from the device
+// selection's state: key / (99.5th percentile), at most 3 key / (99.9th), 1
+// without a positive pixel.  IEEE divisions, as the host computes it.
+__device__ __forceinline__ double exposure_of(const ExposureState& st, double key) {
+    if (st.n_pos == 0ull) return 1.0;
+    const double hi = __longlong_as_double((long long)st.prefix[0]);
+    const double peak = __longlong_as_double((long long)st.prefix[1]);
+    double e = 1.0;
+    if (hi > 0.0) e = key / hi;
+    if (peak > 0.0) e = dmin(e, 3.0 * key / peak);
+    return e;
+}
+
+// `st` non-null: the exposure is exposure_of(*st, key), computed where the
+// selection left it, with no host round trip (and stored to *e_out by the
+// first thread when e_out is non-null); otherwise `exposure` is used.
+__global__ void k_develop(const double* __restrict__ xyz, int64_t n, double exposure,
+                          const ExposureState* __restrict__ st, double key, double* __restrict__ e_out,
+                          int tm, double log_k, uint8_t* __restrict__ rgb) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (st) {
+        exposure = exposure_of(*st, key);
+        if (e_out && i == 0) *e_out = exposure;
+    }
     if (i >= n) return;
     double c0 = xyz[3 * i], c1 = xyz[3 * i + 1], c2 = xyz[3 * i + 2];
     if (!isfinite(c0) || !isfinite(c1) || !isfinite(c2)) c0 = c1 = c2 = 0.0;
@@ -533,7 +554,15 @@ hipError_t launch_shade_batch(const ShadeArgs* table, const int32_t* block_scene
 hipError_t launch_develop(const double* xyz, int64_t n, double exposure, int tm, double log_k,
                           uint8_t* rgb, hipStream_t s) {
     const int blocks = (int)((n + 255) / 256);
-    hipLaunchKernelGGL(k_develop, dim3(blocks), dim3(256), 0, s, xyz, n, exposure, tm, log_k, rgb);
+    hipLaunchKernelGGL(k_develop, dim3(blocks), dim3(256), 0, s, xyz, n, exposure,
+                       static_cast<const ExposureState*>(nullptr), 0.0, static_cast<double*>(nullptr), tm, log_k,
+                       rgb);
+    return hipGetLastError();
+}
+hipError_t launch_develop_auto(const double* xyz, int64_t n, const ExposureState* state, double key,
+                               double* e_out, int tm, double log_k, uint8_t* rgb, hipStream_t s) {
+    const int blocks = (int)std::max<int64_t>(1, (n + 255) / 256);
+    hipLaunchKernelGGL(k_develop, dim3(blocks), dim3(256), 0, s, xyz, n, 1.0, state, key, e_out, tm, log_k, rgb);
     return hipGetLastError();
 }
 hipError_t launch_exposure(const double* xyz, int64_t n, ExposureState* state, unsigned int* hist,

@@ -15,7 +15,7 @@ _lib = None
 P = C.c_void_p
 SYMBOLS = (
     "bhx_create", "bhx_destroy", "bhx_last_error", "bhx_abi_version", "bhx_set_catalogue",
-    "bhx_render", "bhx_trace_dev", "bhx_shade_dev", "bhx_auto_exposure_dev", "bhx_develop_dev", "bhx_add_glare_dev",
+    "bhx_render", "bhx_trace_dev", "bhx_shade_dev", "bhx_auto_exposure_dev", "bhx_develop_dev", "bhx_develop_auto_dev", "bhx_add_glare_dev",
     "bhx_blackbody_xyz_dev", "bhx_set_launch", "bhx_get_launch", "bhx_setup_jet",
     "bhx_polytrope_star", "bhx_setup_neutron_star", "bhx_setup_conical_jet", "bhx_batch_create", "bhx_batch_destroy",
     "bhx_batch_run", "bhx_batch_scene", "bhx_batch_copy_scene", "bhx_batch_stats",
@@ -53,6 +53,7 @@ def lib():
                                 C.POINTER(Stats), P]
     L.bhx_auto_exposure_dev.argtypes = [P, P, C.c_int64, C.c_double, C.POINTER(C.c_double), P]
     L.bhx_develop_dev.argtypes = [P, P, C.c_int64, C.c_double, C.c_int32, C.c_double, P, P]
+    L.bhx_develop_auto_dev.argtypes = [P, P, C.c_int64, C.c_double, C.c_int32, C.c_double, P, P, P]
     L.bhx_add_glare_dev.argtypes = [P, P, C.c_int32, C.c_int32, C.c_double, C.c_double, P]
     L.bhx_blackbody_xyz_dev.argtypes = [P, P, C.c_int64, P, P]
     L.bhx_set_launch.argtypes = [P, C.POINTER(Launch)]

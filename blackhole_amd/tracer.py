@@ -175,6 +175,25 @@ class Tracer:
                      "bhx_develop_dev")
         return rgb.view(-1, 3)
 
+    def develop_auto(self, xyz, key=1.0, tonemap=TM_HDR, log_decades=16.0, exposure_out=None, stream=None):
+        """auto_exposure then develop (src/image.cpp:22-45, 119-191) with the
+        exposure kept on the device: the same sRGB8 as
+        develop(xyz, auto_exposure(xyz, key), ...), queued on the stream with
+        no host synchronisation.  exposure_out: an optional one-element
+        float64 device tensor that receives the exposure."""
+        t = self.torch
+        n = xyz.numel() // 3
+        rgb = t.empty(n * 3, dtype=t.uint8, device=xyz.device)
+        e_ptr = None
+        if exposure_out is not None:
+            if exposure_out.dtype != t.float64 or exposure_out.numel() < 1 or not exposure_out.is_cuda:
+                raise ValueError("exposure_out must be a float64 device tensor")
+            e_ptr = _ptr(exposure_out)
+        native.check(self.ctx, self.lib.bhx_develop_auto_dev(self.ctx, _ptr(xyz), n, key, tonemap, log_decades,
+                                                             _ptr(rgb), e_ptr, self._stream(stream)),
+                     "bhx_develop_auto_dev")
+        return rgb.view(-1, 3)
+
     # ---- PNG / APNG (img::write_png / write_apng, src/image.cpp:193-778) -----
     def _take(self, out):
         try:

@@ -216,7 +216,6 @@ int bhx_shade_dev(bhx_context *ctx, const bhx_scene *scene, const bhx_rays *rays
 int bhx_auto_exposure_dev(bhx_context *ctx, const double *xyz, int64_t n_pixels, double key,
                           double *exposure, void *stream);
 
-/* img::develop (src/image.cpp:119-191): device XYZ frame -> device sRGB8. */
 /* img::add_glare (src/image.cpp:47-106) on a device XYZ image (row-major,
  * width x height x 3 doubles), in place: the image plus a sigma = radius_px
  * and a sigma = 3 radius_px Gaussian of itself, weights 1 - strength,
@@ -226,8 +225,21 @@ int bhx_auto_exposure_dev(bhx_context *ctx, const double *xyz, int64_t n_pixels,
  * before its taps are uploaded). */
 int bhx_add_glare_dev(bhx_context *ctx, double *xyz, int32_t width, int32_t height, double strength,
                       double radius_px, void *stream);
+/* img::develop (src/image.cpp:119-191): device XYZ frame -> device sRGB8. */
 int bhx_develop_dev(bhx_context *ctx, const double *xyz, int64_t n_pixels, double exposure,
                     int32_t tonemap, double log_decades, uint8_t *rgb, void *stream);
+
+/* img::auto_exposure then img::develop (src/image.cpp:22-45, 119-191) without
+ * the host round trip: no counterpart in the reference, whose two calls
+ * return the exposure to the caller in between.  The exposure stays on the
+ * device - the same bits as bhx_auto_exposure_dev gives, the same sRGB8 as
+ * bhx_develop_dev with it - so a frame's trace, shading and display pipeline
+ * are queued on `stream` with no synchronisation.  exposure_dev, when not
+ * NULL, is a device double that receives the exposure (n_pixels > 0).
+ * Stream-ordered, asynchronous; calls that overlap on one context share its
+ * selection buffers and race. */
+int bhx_develop_auto_dev(bhx_context *ctx, const double *xyz, int64_t n_pixels, double key, int32_t tonemap,
+                         double log_decades, uint8_t *rgb, double *exposure_dev, void *stream);
 
 /* spec::blackbody_xyz (src/spectrum.cpp:123-134) of n temperatures, device
  * arrays, through the same 4096-entry table the shader uses. */

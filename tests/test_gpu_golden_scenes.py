@@ -64,6 +64,30 @@ def test_display_transform_matches_reference(tracer):
         assert d.max() <= 1, tm
 
 
+def test_develop_auto_equals_exposure_then_develop(tracer):
+    """The device-side auto exposure + develop: the exposure the two-call
+    path returns to the host, bit for bit, and the same sRGB8 for every tone
+    curve, on the reference's display pixels and on a shaded frame."""
+    import torch
+    frames = [torch.from_numpy(np.ascontiguousarray(ARR["display/xyz"])).cuda()]
+    for name in ("starry_48", "quasar_jets"):
+        sc = scene_from(META["scenes"][name]["scene"])
+        rays, st = tracer.trace(sc)
+        sh, _ = tracer.shade(sc, rays, stats=st)
+        frames.append(sh["xyz"].contiguous())
+    frames.append(torch.zeros(64, 3, dtype=torch.float64, device="cuda"))   # no positive pixel: exposure 1
+    for f, xyz in enumerate(frames):
+        for key in (1.0, 0.18):
+            e = tracer.auto_exposure(xyz, key)
+            for tm in range(5):
+                eo = torch.full((1,), -1.0, dtype=torch.float64, device="cuda")
+                rgb = tracer.develop_auto(xyz, key, tm, 16.0, exposure_out=eo)
+                ref = tracer.develop(xyz, e, tm, 16.0)
+                assert eo.item() == e, (f, key, tm)
+                assert torch.equal(rgb, ref), (f, key, tm)
+        print(f"[develop_auto] frame {f}: {xyz.numel() // 3} px, exposure {e!r}", flush=True)
+
+
 def test_colour_table_matches_reference(tracer):
     """spec::blackbody_xyz through the device table vs the reference's values
     (same table, built on the host; the device log10 may differ by an ulp)."""
