@@ -169,6 +169,8 @@ int bhx_destroy(bhx_context* ctx) {
         b->release();
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    if (ctx->order_free) (void)hipEventDestroy(ctx->order_free);
+    if (ctx->order_stream) (void)hipStreamDestroy(ctx->order_stream);
     delete ctx;
     return BHX_OK;
 }
@@ -275,21 +277,47 @@ int bhx_trace_dev(bhx_context* ctx, const bhx_scene* s, const bhx_rays* rays, bh
         block = 64;
         grid = static_cast<int>((n_rays + 63) / 64);
     }
-    // counters are cleared by a fill kernel (never a memset node: the call may be captured)
-    HIPCHK(launch_zero_u64(reinterpret_cast<unsigned long long*>(ctx->d_cnt), sizeof(Counters) / 8, st));
+    // counters are cleared by fill kernels (never a memset node: the call may
+    // be captured); the tracer's here, the pilot's with the order below
+    Counters* cnt = ctx->d_cnt;
+    HIPCHK(launch_zero_u64(cnt->totals, 4, st));
+    HIPCHK(launch_zero_u64(cnt->classes, 8, st));
+    HIPCHK(launch_zero_u32(&cnt->tile_counter, 1, st));
     // longest-first queue when every resident lane sees several rays (below
     // that the queue hardly drains before the end and the pilot is overhead)
     const bool by_cost = ctx->launch.queue_order == BHX_ORDER_COST && !P.has_jet &&
                          n_rays >= 4 * static_cast<int64_t>(grid) * block;
+    bool order_side = false;
     if (by_cost) {
-        HIPCHK(hipEventRecord(ctx->ev[4], st));
-        rc = cost_order(ctx, G, P, st);
+        // On the order stream unless the call is being captured: it waits only
+        // for the last frame's ray generation to have read the previous order,
+        // so back-to-back frames build it during the last frame's trace (whose
+        // retiring waves free the CUs it needs) and shading.  Same order, same rays.
+        hipStream_t os = st;
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        HIPCHK(hipStreamIsCapturing(st, &cap));
+        if (cap == hipStreamCaptureStatusNone) {
+            if (!ctx->order_stream) {
+                HIPCHK(hipStreamCreateWithFlags(&ctx->order_stream, hipStreamNonBlocking));
+                HIPCHK(hipEventCreateWithFlags(&ctx->order_free, hipEventDisableTiming));
+                HIPCHK(hipEventRecord(ctx->order_free, st));
+            }
+            os = ctx->order_stream;
+            order_side = true;
+            HIPCHK(hipStreamWaitEvent(os, ctx->order_free, 0));
+        }
+        HIPCHK(launch_zero_u32(&cnt->pilot_counter, 1, os));
+        HIPCHK(launch_zero_u64(cnt->pilot_totals, 4, os));
+        HIPCHK(hipEventRecord(ctx->ev[4], os));
+        rc = cost_order(ctx, G, P, os);
         if (rc) return rc;
-        HIPCHK(hipEventRecord(ctx->ev[5], st));
+        HIPCHK(hipEventRecord(ctx->ev[5], os));
+        if (order_side) HIPCHK(hipStreamWaitEvent(st, ctx->ev[5], 0));
         G.tile_order = static_cast<const int32_t*>(ctx->cost_order.p);
         G.tile_start = static_cast<const int32_t*>(ctx->cost_start.p);
     }
     HIPCHK(launch_raygen(G, st));
+    if (order_side) HIPCHK(hipEventRecord(ctx->order_free, st));
     HIPCHK(hipEventRecord(ctx->ev[0], st));
     HIPCHK(launch_trace(P, grid, block, st));
     HIPCHK(hipEventRecord(ctx->ev[1], st));

@@ -109,6 +109,12 @@ struct bhx_context {
     bool ev_valid = false;
     // counters
     Counters* d_cnt = nullptr;
+    // The cost-ordered queue is built on a stream of its own (non-blocking,
+    // default priority): a frame's pilot and sort only wait until the last
+    // frame's ray generation has read the previous order (order_free), so they
+    // run while that frame still traces and shades.
+    hipStream_t order_stream = nullptr;
+    hipEvent_t order_free = nullptr;
     DevBuf flux_partial, exp_hist, exp_state;
     // frame buffers of bhx_render
     DevBuf r_cls, r_y, r_jet, r_xyz;

@@ -178,7 +178,10 @@ int bhx_render(bhx_context *ctx, const bhx_scene *scene, double *xyz, bhx_stats 
  * n_rays = width*height*sqrt_spp^2 entries indexed by ray.  `stats` may be
  * NULL; when given, the call synchronises the stream and accumulates rays,
  * steps, attempts, RHS evaluations and the invariant maxima (the per-class
- * counters are filled by bhx_shade_dev).
+ * counters are filled by bhx_shade_dev).  The cost-ordered queue of a large
+ * frame is built on a stream the context owns, joined to `stream` by events
+ * (on `stream` itself while it is being captured): frames queued back to back
+ * build their order while the previous frame traces and shades.
  */
 int bhx_trace_dev(bhx_context *ctx, const bhx_scene *scene, const bhx_rays *rays,
                   bhx_stats *stats, void *stream);

@@ -52,3 +52,35 @@ def test_small_frame_keeps_centre_order(tracer):
     sc = S.plain_frame()
     _, st = tracer.trace(sc)
     assert st.order_ms == 0.0
+
+
+def test_back_to_back_frames_with_the_order_on_its_own_stream(tracer):
+    """Frames queued back to back without a host synchronisation: each
+    frame's cost order is built on the context's order stream while the
+    previous frame still traces, and must be that frame's own (same rays, to
+    the bit, as the frame traced alone).  Three scene shapes in turn, twice
+    over, so an order built for the wrong frame or read after it was replaced
+    would show."""
+    torch = tracer.torch
+    tracer.set_launch(queue_order=ORDER_COST)
+    scenes = [S.starry_frame(1024, 1024, stars=False), S.starry_frame(1000, 700, stars=False),
+              S.starry_frame(1024, 600, stars=False)]
+    alone = []
+    for sc in scenes:
+        rays, st = tracer.trace(sc)
+        assert st.order_ms > 0.0   # every shape here is cost-ordered
+        alone.append(rays)
+    queued = []
+    for rep in range(2):
+        for sc in scenes:
+            rays = tracer.alloc_rays(sc)
+            for k, v in rays.items():
+                v.fill_(255 if k == "cls" else -7)
+            tracer.trace_async(sc, rays)
+            queued.append(rays)
+    torch.cuda.synchronize()
+    for i, rays in enumerate(queued):
+        ref = alone[i % len(scenes)]
+        for k in KEYS:
+            assert torch.equal(rays[k], ref[k]), (i, k)
+    print(f"[queue order] {len(queued)} back-to-back frames equal to the frames traced alone", flush=True)
